@@ -1,0 +1,187 @@
+/*
+ * clv.h -- C ABI of the MI355X complex-Langevin engine (libclv.so).
+ *
+ * The reference (DoAmann/CL_var_dt) has no plugin or FFI boundary: its host
+ * driver calls C++ classes in one translation unit.  Each entry point below
+ * replaces one piece of that class surface; the reference member it stands in
+ * for is cited beside it.  Plain C, no exceptions across the boundary, status
+ * is an int (CLV_OK == 0, negative on error), the message of the last failure
+ * is returned by clv_last_error().
+ *
+ * Field layout at the boundary is the reference's device layout
+ * (Lattice.cu:2-5): complex128 interleaved re/im, index
+ *   ((((c*Nt + tau)*X + x)*Y + y)*Z + z),  c = component 0..3.
+ * Internally the library keeps its own layout (chain-major, FP64 or FP32).
+ *
+ * Threading: a context owns its device memory and one HIP stream.  Calls on
+ * one context must not overlap; separate contexts may share a device.
+ * clv_propagate, clv_propagate_plain's device work and clv_evaluate are
+ * asynchronous to the host; getters
+ * synchronize the context's stream.  The library never prints.
+ */
+#ifndef CLV_H_
+#define CLV_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CLV_OK 0
+#define CLV_ERR_ARG (-1)         /* bad argument / unsupported shape        */
+#define CLV_ERR_HIP (-2)         /* HIP runtime error                       */
+#define CLV_ERR_NOMEM (-3)       /* device allocation failed                */
+#define CLV_ERR_BUDGET (-4)      /* request does not fit the HBM budget     */
+#define CLV_ERR_STATE (-5)       /* call not valid in the current state     */
+#define CLV_ERR_NOTFOUND (-6)    /* unknown request / chain id              */
+
+#define CLV_FP64 0
+#define CLV_FP32 1
+
+/* Q3 of the reference: the psi-bar drift's potential term multiplies
+ * psi(tau+1) (Propagator_without_cutoff.cu:71,88,105,122).  CLV_Q3_REFERENCE
+ * keeps that; CLV_Q3_PSIBAR uses psibar(tau+1) (the action derivative). */
+#define CLV_Q3_REFERENCE 0
+#define CLV_Q3_PSIBAR 1
+
+/* One chain's lattice: X x Y x Z space, Nt imaginary-time slices, 4 comps.
+ * Stands in for the XSIZE/YSIZE/ZSIZE/TAUSIZE macros (Parameters.h:8-11). */
+typedef struct clv_shape {
+  int32_t nx, ny, nz, nt;
+} clv_shape;
+
+/* Per-chain physics and controller parameters.  The reference takes MU and P
+ * as run-time arguments of Propagator::propagate (Propagator_without_cutoff.cu:406)
+ * and everything else as macros (Parameters.h:13-31); here all are run time. */
+typedef struct clv_params {
+  double mu;           /* chemical potential MU (params.txt mu)            */
+  double P;            /* linear Zeeman P (params.txt P)                    */
+  double G;            /* density coupling c0 (Parameters.h:15 G)           */
+  double c1;           /* spin coupling; 0 reproduces the reference         */
+  double Q;            /* quadratic Zeeman (Parameters.h:24 Q)              */
+  double eps;          /* EPS = beta / Nt (Parameters.h:13)                 */
+  double a_x;          /* lattice spacing a_x (Parameters.h:14)             */
+  double DT;           /* controller base step DT (Parameters.h:29)         */
+  double dt_min;       /* lower clamp, 1e-8 (Propagator_without_cutoff.cu:391) */
+  double dt_max;       /* upper clamp, 1e-2 (Propagator_without_cutoff.cu:392) */
+  double alpha;        /* blend, 0.9 (Propagator_without_cutoff.cu:402)     */
+  double factor;       /* drift prefactor, 1 in propagate (:427)            */
+  uint64_t seed;       /* Philox key of this chain's noise stream           */
+  int32_t kin_dims;    /* 1: p^2 = p_x^2 (reference, :196); 3: px^2+py^2+pz^2 */
+  int32_t q3_mode;     /* CLV_Q3_REFERENCE or CLV_Q3_PSIBAR                 */
+  int32_t noise_on;    /* 1: propagate (:406); 0: propagate_without_noise (:444) */
+  int32_t reserved;
+} clv_params;
+
+/* Controller state of one chain after its last step. */
+typedef struct clv_step_stats {
+  double dt;           /* dt used by the last step                          */
+  double max_drift;    /* max |D_0| of the last step                        */
+  double mean_drift;   /* sum |D_0| / (4 V_s Nt) of the last step           */
+  double t_langevin;   /* sum of dt over all steps                          */
+  int64_t steps;       /* steps taken                                       */
+  int64_t clamp_low;   /* steps whose dt hit the lower clamp                */
+  int64_t clamp_high;  /* steps whose dt hit the upper clamp                */
+  int32_t nonfinite;   /* 1 once a non-finite drift has been seen           */
+  int32_t reserved;
+} clv_step_stats;
+
+/* Running observable means, in the form the reference writes them
+ * (Observables_without_cutoff.cu:1527-1547, 1614-1634): sums divided by
+ * (Nt * count) and, for the numbers, by V_s as in write_n_tot. */
+typedef struct clv_observables {
+  int64_t count;       /* number of evaluations (Observables::get_counter)  */
+  double n_tot;        /* total particle number (write_n_tot)               */
+  double n_comp[4];    /* n_11, n_12, n_21, n_22                            */
+  double energy;       /* <H> without the -mu N term (new estimator)        */
+  double Fx_tot, Fy_tot, Fz_tot;  /* sum_x of F_a(x)                        */
+  double density;      /* n_tot / V_s                                       */
+  double Sz;           /* Fz_tot / V_s                                      */
+} clv_observables;
+
+typedef struct clv_ctx clv_ctx;
+
+/* ---- library ---------------------------------------------------------- */
+const char *clv_version(void);
+/* Message of the last failed call on ctx (or of the last failed create when
+ * ctx is NULL).  Never NULL. */
+const char *clv_last_error(const clv_ctx *ctx);
+/* Reference defaults: Parameters.h:13-31 and Propagator_without_cutoff.cu:382-404. */
+void clv_default_params(clv_params *p);
+/* Propagator::compute_new_dt (Propagator_without_cutoff.cu:382-404), on the
+ * host.  clamp: 0 none, -1 low clamp applied, +1 high clamp applied. */
+double clv_compute_new_dt(double mean_drift, double max_drift,
+                          const clv_params *p, int32_t *clamp);
+
+/* ---- context: replaces `new ComplexLattice` x4 + `new Propagator`
+ * (main.cu:94-103, Lattice.cu:108-125, Propagator_without_cutoff.cu:316-343) */
+int clv_create(const clv_shape *shapes, int32_t nchains, int32_t precision,
+               int32_t device, clv_ctx **out);
+int clv_destroy(clv_ctx *ctx);
+int clv_num_chains(const clv_ctx *ctx);
+/* Bytes of device memory the context holds. */
+int64_t clv_device_bytes(const clv_ctx *ctx);
+
+int clv_set_params(clv_ctx *ctx, int32_t chain, const clv_params *p);
+
+/* Fields in the reference layout (see top).  psi / psibar each hold
+ * 2*4*V_s*Nt doubles.  Replaces ComplexLattice::set_mean_field / operator=
+ * (Lattice.cu:190-214) for initialisation and readback. */
+int clv_set_fields(clv_ctx *ctx, int32_t chain, const double *psi,
+                   const double *psibar);
+int clv_get_fields(clv_ctx *ctx, int32_t chain, double *psi, double *psibar);
+/* psi_c = sqrt(n0) + xi_c, xi_c complex Gaussian with variance `var` per
+ * real part from the chain's Philox key (init stream); psibar = conj(psi). */
+int clv_init_fields(clv_ctx *ctx, int32_t chain, double n0, double var);
+
+/* ---- the per-step update: Propagator::propagate (Propagator_without_cutoff.cu:406-442)
+ * Device-resident controller: every chain advances nsteps steps, dt per
+ * chain computed on the device from that chain's own drift.  eval_every > 0
+ * accumulates observables after step i when (chain's step index i) %
+ * eval_every == 0, as main.cu:152-156 does after propagate. */
+int clv_propagate(clv_ctx *ctx, int64_t nsteps, int32_t eval_every);
+
+/* Plain method: the reference's host-driven step.  Per step: drift + max/mean
+ * reduction on the device, a blocking device->host copy of (max, sum), dt on
+ * the host (clv_compute_new_dt), then the update with dt passed by value
+ * (Propagator_without_cutoff.cu:427-441).  Single chain (chain index). */
+int clv_propagate_plain(clv_ctx *ctx, int32_t chain, int64_t nsteps,
+                        int32_t eval_every);
+
+/* Building blocks of the plain step for a host driver that mirrors
+ * Propagator::propagate itself:
+ *   clv_drift_stats  = calculate_drift_without_noise + get_max_and_mean_drift
+ *                      (Propagator_without_cutoff.cu:427-429, 351-380)
+ *   clv_apply_update = multiply_by_timestep + add_noise + fields += drift
+ *                      with the given dt (Propagator_without_cutoff.cu:432-441) */
+int clv_drift_stats(clv_ctx *ctx, int32_t chain, double *max_drift,
+                    double *mean_drift);
+int clv_apply_update(clv_ctx *ctx, int32_t chain, double dt);
+
+int clv_get_step_stats(clv_ctx *ctx, int32_t chain, clv_step_stats *out);
+int clv_synchronize(clv_ctx *ctx);
+
+/* ---- observables: Observables::evaluate / write_* (Observables_without_cutoff.cu:1155-1524, 1527-1634) */
+int clv_evaluate(clv_ctx *ctx, int32_t chain);   /* chain < 0: all chains */
+/* Fx, Fy, Fz: V_s doubles each in (x,y,z) order, written form; may be NULL. */
+int clv_read_observables(clv_ctx *ctx, int32_t chain, clv_observables *out,
+                         double *Fx, double *Fy, double *Fz);
+int clv_reset_observables(clv_ctx *ctx, int32_t chain);
+
+/* ---- kernel selection (project autotuner) ------------------------------ */
+/* Times the candidate kernel schedules of every shape bucket of ctx on its
+ * current state (state is saved and restored) and keeps the fastest.  When
+ * cache_path is non-NULL the choice is read from / appended to that JSON-lines
+ * file, keyed by shape, precision and device name. */
+int clv_autotune(clv_ctx *ctx, const char *cache_path);
+/* Force a schedule: 0 = auto, 1 = two-pass (reduce then update),
+ * 2 = fused single pass (update + next step's reduction). */
+int clv_set_schedule(clv_ctx *ctx, int32_t schedule);
+int clv_get_schedule(const clv_ctx *ctx, int32_t chain);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CLV_H_ */

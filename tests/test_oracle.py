@@ -1,0 +1,332 @@
+"""Pins of the FP64 CPU oracle (CPU only).
+
+Each check is chosen so that a plausible mistake (a dropped term, a wrong sign
+or index, a transposed operand) fails one of them: published Philox KATs,
+numpy.fft and brute-force DFTs, plane-wave eigenvalues of the kinetic term,
+-dS/dpsibar of an independently written action, the dt controller's closed
+form, the noise's conjugation relation and moments, Parseval for the
+observables and hand values on constant fields.
+"""
+import numpy as np
+import pytest
+
+from cl_var_dt_amd import CLV_Q3_PSIBAR, CLV_Q3_REFERENCE, Shape, default_params
+from tests import physics_np as ph
+
+
+# ---------------------------------------------------------------- Philox ----
+# Known-answer vectors of Philox4x32-10 (Salmon et al., SC'11; Random123 kat_vectors).
+KAT = [
+    ((0, 0, 0, 0), (0, 0), (0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8)),
+    ((0xffffffff,) * 4, (0xffffffff,) * 2, (0x408f276d, 0x41c83b0e, 0xa20bc7c6, 0x6d5451fd)),
+    ((0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344), (0xa4093822, 0x299f31d0),
+     (0xd16cfe09, 0x94fdcceb, 0x5001e420, 0x24126ea1)),
+]
+
+
+@pytest.mark.parametrize("ctr,key,want", KAT)
+def test_philox_kat(oracle_lib, ctr, key, want):
+    got = oracle_lib.philox(ctr, key)
+    assert [int(v) for v in got] == list(want)
+
+
+def test_normals_moments_and_independence(oracle_lib):
+    # 4 * 20000 normals from distinct sites / pairs / steps / streams
+    vals = []
+    for site in range(5000):
+        vals.append(oracle_lib.normals(12345, 0, 7, site, site & 1))
+    v = np.array(vals).ravel()
+    assert abs(v.mean()) < 4 / np.sqrt(v.size)
+    assert abs(v.var() - 1.0) < 5 * np.sqrt(2.0 / v.size)
+    # fourth moment of a standard normal is 3
+    assert abs(np.mean(v ** 4) - 3.0) < 5 * np.sqrt(96.0 / v.size)
+    # streams, steps, pairs and keys give different draws
+    a = oracle_lib.normals(1, 0, 0, 0, 0)
+    for other in [(2, 0, 0, 0, 0), (1, 1, 0, 0, 0), (1, 0, 1, 0, 0), (1, 0, 0, 1, 0), (1, 0, 0, 0, 1)]:
+        assert not np.allclose(a, oracle_lib.normals(*other))
+    # 64-bit step counter: the high word matters
+    assert not np.allclose(oracle_lib.normals(1, 0, 1, 0, 0), oracle_lib.normals(1, 0, 1 + (1 << 32), 0, 0))
+
+
+def test_normals_distribution_ks(oracle_lib):
+    from scipy import stats
+    v = np.array([oracle_lib.normals(99, 0, 3, s, 0) for s in range(3000)]).ravel()
+    assert stats.kstest(v, "norm").pvalue > 1e-3
+
+
+# ------------------------------------------------------------------- FFT ----
+def brute_dft(x, sign):
+    n = x.size
+    k = np.arange(n)
+    return np.exp(sign * 2j * np.pi * np.outer(k, k) / n) @ x
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 4, 5, 6, 7, 8, 9, 12, 15, 16, 24, 32, 48, 64, 96, 128, 256])
+@pytest.mark.parametrize("sign", [-1, 1])
+def test_fft1d(oracle_lib, n, sign):
+    rng = np.random.default_rng(n)
+    x = rng.normal(size=n) + 1j * rng.normal(size=n)
+    got = oracle_lib.fft1d(x, sign)
+    ref = np.fft.fft(x) if sign < 0 else np.fft.ifft(x) * n
+    assert np.allclose(got, ref, rtol=0, atol=1e-12 * n)
+    if n <= 64:
+        assert np.allclose(got, brute_dft(x, sign), rtol=0, atol=1e-12 * n)
+
+
+@pytest.mark.parametrize("dims", [(8, 1, 1), (4, 6, 1), (16, 16, 1), (6, 4, 8), (24, 24, 24)])
+def test_fft3d_matches_numpy_and_roundtrip(oracle_lib, dims):
+    rng = np.random.default_rng(sum(dims))
+    x = rng.normal(size=dims) + 1j * rng.normal(size=dims)
+    got = oracle_lib.fft3d(x, -1)
+    assert np.allclose(got, np.fft.fftn(x), atol=1e-11 * x.size)
+    back = oracle_lib.fft3d(got, +1)
+    assert np.allclose(back, x * x.size, atol=1e-10 * x.size)
+    # Parseval
+    assert np.isclose(np.sum(abs(got) ** 2), x.size * np.sum(abs(x) ** 2), rtol=1e-12)
+
+
+# --------------------------------------------------------------- kinetic ----
+def _oracle(oracle_lib, dims, **kw):
+    X, Y, Z, Nt = dims
+    p = default_params(**kw)
+    return oracle_lib.Oracle(Shape(X, Y, Z, Nt), p, nthreads=2), p
+
+
+@pytest.mark.parametrize("dims", [(16, 1, 1, 4), (8, 4, 1, 2), (6, 4, 2, 2)])
+def test_kinetic_plane_waves(oracle_lib, dims):
+    X, Y, Z, Nt = dims
+    o, p = _oracle(oracle_lib, dims, a_x=0.7)
+    x = np.arange(X)[:, None, None]
+    y = np.arange(Y)[None, :, None]
+    for j0 in range(X):
+        px = ph.momenta(X)[j0]
+        if X % 2 == 0 and j0 == X // 2:
+            assert px == pytest.approx(-np.pi)    # Nyquist convention (:169-176)
+        wave = np.exp(1j * 2 * np.pi * j0 * x / X) * np.ones((X, Y, Z))
+        f = np.broadcast_to(wave, (4, Nt, X, Y, Z)).copy()
+        o.set_fields(f, np.conj(f))
+        K, Kb = o.kinetic()
+        assert np.allclose(K, -px ** 2 * f / 0.7, atol=1e-12)
+        assert np.allclose(Kb, -px ** 2 * np.conj(f) / 0.7, atol=1e-12)
+    # Q1: a wave along y alone is annihilated (p^2 = p_x^2 only)
+    if Y > 1:
+        wave = np.exp(2j * np.pi * y / Y) * np.ones((X, Y, Z))
+        f = np.broadcast_to(wave, (4, Nt, X, Y, Z)).copy()
+        o.set_fields(f, f)
+        K, _ = o.kinetic()
+        assert np.allclose(K, 0, atol=1e-12)
+        # kin_dims = 3: full Laplacian
+        o.set_params(default_params(a_x=0.7, kin_dims=3))
+        o.set_fields(f, f)
+        K, _ = o.kinetic()
+        assert np.allclose(K, -(2 * np.pi / Y) ** 2 * f / 0.7, atol=1e-12)
+
+
+def test_kinetic_matches_numpy(oracle_lib):
+    dims = (8, 4, 2, 3)
+    o, p = _oracle(oracle_lib, dims)
+    rng = np.random.default_rng(0)
+    sh = (4, 3, 8, 4, 2)
+    f = rng.normal(size=sh) + 1j * rng.normal(size=sh)
+    g = rng.normal(size=sh) + 1j * rng.normal(size=sh)
+    o.set_fields(f, g)
+    K, Kb = o.kinetic()
+    assert np.allclose(K, ph.kinetic(f), atol=1e-12)
+    assert np.allclose(Kb, ph.kinetic(g), atol=1e-12)
+
+
+# ----------------------------------------------------------------- drift ----
+def _rand_fields(rng, sh, scale=0.6):
+    f = 1.0 + scale * (rng.normal(size=sh) + 1j * rng.normal(size=sh))
+    g = 1.0 + scale * (rng.normal(size=sh) + 1j * rng.normal(size=sh))
+    return f, g
+
+
+@pytest.mark.parametrize("c1", [0.0, 0.07, -0.05])
+@pytest.mark.parametrize("dims", [(4, 1, 1, 4), (4, 2, 1, 3)])
+def test_drift_is_minus_action_gradient(oracle_lib, c1, dims):
+    X, Y, Z, Nt = dims
+    kw = dict(mu=0.8, P=0.13, Q=0.21, G=0.3, c1=c1, eps=0.07, a_x=1.0, q3_mode=CLV_Q3_PSIBAR)
+    o, p = _oracle(oracle_lib, dims, **kw)
+    rng = np.random.default_rng(42)
+    sh = (4, Nt, X, Y, Z)
+    f, g = _rand_fields(rng, sh)
+    o.set_fields(f, g)
+    D, Db = o.drift()
+    for _ in range(12):
+        idx = tuple(int(rng.integers(0, n)) for n in sh)
+        dS_db = ph.d_action(f, g, p, "psibar", idx)
+        dS_dq = ph.d_action(f, g, p, "psi", idx)
+        assert abs(D[idx] + dS_db) < 1e-10 * max(1, abs(dS_db)), idx
+        assert abs(Db[idx] + dS_dq) < 1e-10 * max(1, abs(dS_dq)), idx
+
+
+def test_drift_q3_reference_mode(oracle_lib):
+    """Q3: in the reference the psibar drift's s_i term multiplies psi(t+1)."""
+    dims = (8, 1, 1, 4)
+    kw = dict(mu=0.6, P=0.2, Q=0.1, G=0.2, eps=0.05, a_x=1.3, factor=0.7)
+    o_ref, _ = _oracle(oracle_lib, dims, q3_mode=CLV_Q3_REFERENCE, **kw)
+    o_fix, _ = _oracle(oracle_lib, dims, q3_mode=CLV_Q3_PSIBAR, **kw)
+    rng = np.random.default_rng(3)
+    f, g = _rand_fields(rng, (4, 4, 8, 1, 1))
+    for o in (o_ref, o_fix):
+        o.set_fields(f, g)
+    D1, Db1 = o_ref.drift()
+    D2, Db2 = o_fix.drift()
+    assert np.array_equal(D1, D2)
+    s = np.array([0.6 + 0.2 + 0.1, 0.6, 0.6, 0.6 - 0.2 + 0.1])[:, None, None, None, None]
+    diff = 1.3 * 0.7 * 0.05 * s * (np.roll(f, -1, axis=1) - np.roll(g, -1, axis=1))
+    assert np.allclose(Db1 - Db2, diff, atol=1e-13)
+
+
+def test_drift_special_cases(oracle_lib):
+    dims = (8, 2, 1, 4)
+    sh = (4, 4, 8, 2, 1)
+    o, p = _oracle(oracle_lib, dims, mu=0.9, G=0.0, P=0.0, Q=0.0, eps=0.1, a_x=1.0)
+    z = np.zeros(sh, complex)
+    o.set_fields(z, z)
+    D, Db = o.drift()
+    assert not D.any() and not Db.any()
+    # G = P = Q = 0 and fields constant in space: D = psi(t-1)(1 + eps mu) - psi(t)
+    rng = np.random.default_rng(5)
+    ft = rng.normal(size=(4, 4)) + 1j * rng.normal(size=(4, 4))
+    f = np.broadcast_to(ft[:, :, None, None, None], sh).copy()
+    o.set_fields(f, np.conj(f))
+    D, _ = o.drift()
+    want = np.roll(f, 1, axis=1) * (1 + 0.1 * 0.9) - f
+    assert np.allclose(D, want, atol=1e-14)
+    # uniform fields psi = psibar = v: T_i = 4 v^3 (three n's plus the triple), D = eps(mu v - 8 G v^3)
+    o2, _ = _oracle(oracle_lib, dims, mu=0.5, G=0.2, eps=0.1, a_x=1.0, q3_mode=CLV_Q3_PSIBAR)
+    v = 0.9 + 0.2j
+    f = np.full(sh, v)
+    o2.set_fields(f, f)
+    D, Db = o2.drift()
+    assert np.allclose(D, 0.1 * (0.5 * v - 2 * 0.2 * 4 * v ** 3), atol=1e-14)
+    assert np.allclose(Db, D, atol=1e-14)
+
+
+# ---------------------------------------------------------- dt controller ----
+def test_new_dt_table(oracle_lib):
+    p = default_params(DT=0.002)
+    # max <= 0 -> DT, blended -> DT
+    assert oracle_lib.new_dt(0.0, 0.0, p) == (pytest.approx(0.002, abs=0), 0)
+    # ordinary: DT mean/max blended
+    dt, c = oracle_lib.new_dt(0.05, 0.5, p)
+    assert c == 0 and dt == pytest.approx(0.1 * 0.002 + 0.9 * 0.0002, rel=1e-15)
+    # low clamp
+    dt, c = oracle_lib.new_dt(1e-12, 1.0, p)
+    assert c == -1 and dt == pytest.approx(0.1 * 0.002 + 0.9 * 1e-8, rel=1e-15)
+    # high clamp (needs mean/max * DT > 1e-2)
+    dt, c = oracle_lib.new_dt(10.0, 1.0, p)
+    assert c == 1 and dt == pytest.approx(0.1 * 0.002 + 0.9 * 1e-2, rel=1e-15)
+
+
+def test_max_mean_is_component0_over_4S(oracle_lib):
+    dims = (8, 2, 1, 4)
+    o, p = _oracle(oracle_lib, dims, mu=0.3, G=0.1, eps=0.05)
+    rng = np.random.default_rng(8)
+    f, g = _rand_fields(rng, (4, 4, 8, 2, 1))
+    o.set_fields(f, g)
+    D, _ = o.drift()
+    mx, mean = o.max_mean()
+    a0 = np.abs(D[0])
+    assert mx == pytest.approx(a0.max(), rel=1e-15)
+    assert mean == pytest.approx(a0.sum() / D.size, rel=1e-13)   # Q2: / (4 V_s Nt)
+
+
+# ---------------------------------------------------------------- update ----
+def test_update_noise_conjugation_and_stream(oracle_lib):
+    dims = (8, 2, 1, 4)
+    sh = (4, 4, 8, 2, 1)
+    seed = 0x1234_5678_9ABC
+    o, p = _oracle(oracle_lib, dims, mu=0.4, G=0.1, eps=0.05, seed=seed)
+    rng = np.random.default_rng(9)
+    f, g = _rand_fields(rng, sh)
+    o.set_fields(f, g)
+    D, Db = o.drift()
+    dt = 1.7e-3
+    o.apply_update(dt)
+    f1, g1 = o.get_fields()
+    eta = f1 - f - dt * D
+    etab = g1 - g - dt * Db
+    assert np.allclose(etab, np.conj(eta), atol=1e-14)        # Q5
+    # eta at (c, t, x, y, z) = sqrt(dt) * normal pair of (seed, 0, step 0, site, c // 2)
+    for (c, t, x, y, z) in [(0, 0, 0, 0, 0), (1, 3, 7, 1, 0), (2, 1, 4, 0, 0), (3, 2, 5, 1, 0)]:
+        site = ((t * 8 + x) * 2 + y) * 1 + z
+        nz = oracle_lib.normals(seed, 0, 0, site, c // 2)
+        want = np.sqrt(dt) * (nz[2 * (c % 2)] + 1j * nz[2 * (c % 2) + 1])
+        assert abs(eta[c, t, x, y, z] - want) < 1e-14
+    assert np.var(eta.real) / dt == pytest.approx(1.0, abs=0.25)
+    # noise-free update is exactly psi + dt D
+    o2, _ = _oracle(oracle_lib, dims, mu=0.4, G=0.1, eps=0.05, noise_on=0)
+    o2.set_fields(f, g)
+    o2.apply_update(dt)
+    f2, g2 = o2.get_fields()
+    assert np.allclose(f2, f + dt * D, atol=1e-15) and np.allclose(g2, g + dt * Db, atol=1e-15)
+
+
+def test_propagate_uses_controller_and_is_deterministic(oracle_lib):
+    dims = (16, 1, 1, 8)
+    kw = dict(mu=1.0, G=0.1, c1=-0.05, Q=0.1, eps=1 / 8, DT=1e-3, seed=1)
+    runs = []
+    for _ in range(2):
+        o, p = _oracle(oracle_lib, dims, **kw)
+        o.init_fields(1.0, 0.1)
+        mx, mean = o.max_mean()
+        dt_expect, _ = oracle_lib.new_dt(mean, mx, p)
+        o.propagate(1)
+        st = o.stats()
+        assert st.dt == dt_expect and st.steps == 1
+        assert st.max_drift == mx and st.mean_drift == mean
+        o.propagate(20)
+        runs.append(o.get_fields())
+        assert o.stats().steps == 21
+    assert np.array_equal(runs[0][0], runs[1][0])
+
+
+def test_init_fields(oracle_lib):
+    dims = (32, 2, 1, 8)
+    o, p = _oracle(oracle_lib, dims, seed=77)
+    o.init_fields(1.0, 0.1)
+    f, g = o.get_fields()
+    assert np.array_equal(g, np.conj(f))
+    xi = f - 1.0
+    assert abs(xi.mean()) < 0.03
+    assert np.var(xi.real) == pytest.approx(0.1, rel=0.1)
+    assert np.var(xi.imag) == pytest.approx(0.1, rel=0.1)
+
+
+# ----------------------------------------------------------- observables ----
+def test_observables_parseval_and_hand_values(oracle_lib):
+    dims = (8, 2, 1, 4)
+    sh = (4, 4, 8, 2, 1)
+    o, p = _oracle(oracle_lib, dims, mu=0.5, P=0.1, Q=0.2, G=0.3, c1=0.05, a_x=1.0)
+    rng = np.random.default_rng(11)
+    f, g = _rand_fields(rng, sh)
+    o.set_fields(f, g)
+    sums = o.observable_sums()
+    # k-space n (evaluate_n_*) = V_s * real-space n
+    assert np.allclose(sums[:4], 16 * sums[4:8], rtol=1e-12)
+    want_n = [np.sum((np.roll(g, -1, axis=1) * f)[c]).real for c in range(4)]
+    assert np.allclose(sums[4:8], want_n, rtol=1e-12)
+    # energy = sum Re h(psibar(t+1), psi(t)) without the mu term
+    e_want = np.sum(ph.h_density(np.roll(g, -1, axis=1), f, p, include_mu=False)).real
+    assert sums[8] == pytest.approx(e_want, rel=1e-12)
+    # constant fields: hand values of the written observables
+    v = np.array([0.9 + 0.1j, 0.4 - 0.2j, 0.3 + 0.3j, 0.7 + 0.0j])
+    f = np.broadcast_to(v[:, None, None, None, None], sh).copy()
+    o.reset_observables()
+    o.set_fields(f, np.conj(f))
+    o.evaluate()
+    o.evaluate()
+    obs, F = o.observables()
+    n = np.abs(v) ** 2
+    assert obs.count == 2
+    # written numbers are totals over the V_s = 16 sites (write_n_*, :1527-1547)
+    assert np.allclose(list(obs.n_comp), 16 * n, rtol=1e-13)
+    assert obs.n_tot == pytest.approx(16 * n.sum(), rel=1e-13)
+    assert obs.density == pytest.approx(n.sum(), rel=1e-13)
+    assert np.allclose(F[2], n[0] + n[1] - n[2] - n[3], rtol=1e-13)
+    fx = 2 * (np.conj(v[2]) * v[0] + np.conj(v[3]) * v[1]).real
+    assert np.allclose(F[0], fx, rtol=1e-13)
+    assert obs.Sz == pytest.approx(n[0] + n[1] - n[2] - n[3], rel=1e-13)
