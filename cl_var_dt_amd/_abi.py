@@ -26,6 +26,9 @@ SCHED_AUTO = 0
 SCHED_TWO_PASS = 1
 SCHED_FUSED = 2
 
+KERNEL_KINDS = {"drift_stats": 0, "controller": 1, "update": 2, "obs": 3, "obs_finish": 4,
+                "fused": 5}
+
 
 class Shape(C.Structure):
     """clv_shape: one chain's X x Y x Z x Nt lattice (Parameters.h:8-11)."""
@@ -125,6 +128,8 @@ def _declare(lib):
         "clv_autotune": (C.c_int, [vp, C.c_char_p]),
         "clv_set_schedule": (C.c_int, [vp, i32]),
         "clv_get_schedule": (C.c_int, [vp, i32]),
+        "clv_set_profiling": (C.c_int, [vp, i32]),
+        "clv_kernel_time": (C.c_int, [vp, i32, P(dbl), P(i64)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -142,6 +147,7 @@ def exported_symbols() -> list[str]:
         "clv_propagate", "clv_propagate_plain", "clv_drift_stats", "clv_apply_update",
         "clv_get_step_stats", "clv_synchronize", "clv_evaluate", "clv_read_observables",
         "clv_reset_observables", "clv_autotune", "clv_set_schedule", "clv_get_schedule",
+        "clv_set_profiling", "clv_kernel_time",
     ]
 
 

@@ -1,0 +1,1186 @@
+// clv_engine.hip -- gfx950 kernels of the complex-Langevin step and the C ABI
+// of include/clv.h.
+//
+// One Langevin step of a bucket of equal-shape chains (two-pass schedule):
+//   k_drift_stats  K_0(t-1) by an LDS row transform, D_0, per-tile max/sum of |D_0|
+//   k_controller   fixed-order per-chain reduction of the tiles, dt (device) or
+//                  stats for the host (plain method), bookkeeping
+//   k_update       K(t-1), Kbar(t+1) for all components, D and Dbar, Philox
+//                  noise in registers, psi/psibar written to the other buffer
+// Observables (every eval_every steps): k_obs (per tau-chunk) + k_obs_finish.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <map>
+#include <memory>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "../../include/clv.h"
+#include "clv_kernels.h"
+
+using namespace clv;
+
+// ============================================================== kernels ====
+
+// D_0 at every site of one tau-slice of `rows` rows; writes the tile's
+// (max |D_0|, sum |D_0|) -- compute_max_and_mean_drift, Propagator_without_cutoff.cu:231-265,
+// in FP64 and without atomics.
+template <typename T, int X>
+__global__ __launch_bounds__(256) void k_drift_stats(BucketArgs<T> A) {
+  extern __shared__ __align__(16) char smem_raw[];
+  const int chain = A.chain0 + blockIdx.z;
+  const ChainParams &P = A.prm[chain];
+  if (!P.active) return;
+  const int tile = blockIdx.x, t = blockIdx.y;
+  const int tid = threadIdx.x, nthr = blockDim.x;
+  const int j = tid / X, x = tid - (tid / X) * X;
+  const bool has_site = j < A.rows;
+  Smem<T, X> sm(smem_raw, A.rows);
+  load_tables<T, X>(sm, A.tw, P.a_x, tid, nthr);
+  const int par = A.st[chain].parity;  // current buffer
+  const C2<T> *psi = (par ? A.buf1_psi : A.buf0_psi) + chain * A.chain_stride;
+  const C2<T> *psib = (par ? A.buf1_psib : A.buf0_psib) + chain * A.chain_stride;
+  const int r = tile * A.rows + j;
+  const int tm1 = (t == 0) ? A.Nt - 1 : t - 1;
+  C2<T> pm[4], b0[4], p00 = {T(0), T(0)};
+  if (has_site) {
+    load4(psi, (int64_t(tm1) * A.R + r) * X + x, pm);
+    load4(psib, (int64_t(t) * A.R + r) * X + x, b0);
+    p00 = load1(psi, (int64_t(t) * A.R + r) * X + x, 0);
+    sm.a[j * X + x] = pm[0];
+  }
+  __syncthreads();
+  const C2<T> *K = kinetic_lines<T, X>(sm.a, sm.b, sm.tw, sm.mult, A.rows, tid, nthr);
+  double v = 0.0;
+  if (has_site) {
+    const Coef<T> k = coef<T>(P);
+    const C2<T> d = drift0_site(k, pm, p00, b0, K[j * X + x]);
+    const double dx = double(d.x), dy = double(d.y);
+    v = sqrt(dx * dx + dy * dy);
+  }
+  double m = wave_max(v), s = wave_sum(v);
+  const int w = tid >> 6, nw = (nthr + 63) >> 6;
+  if ((tid & 63) == 0) {
+    sm.red[w] = m;
+    sm.red[32 + w] = s;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    for (int q = 1; q < nw; ++q) {
+      m = fmax(m, sm.red[q]);
+      s += sm.red[32 + q];
+    }
+    double *o = A.partials + ((int64_t(chain) * A.Nt + t) * A.ntiles + tile) * 2;
+    o[0] = m;
+    o[1] = s;
+  }
+}
+
+// Per-chain controller: one wave per chain reduces the tiles' partials in a
+// fixed order.  mode 0: dt on the device (compute_new_dt, :382-404) and commit;
+// mode 1: only publish max/mean (the plain method reads them on the host);
+// mode 2: commit a dt chosen on the host (clamp code in host_clamp).
+template <typename T>
+__global__ __launch_bounds__(64) void k_controller(BucketArgs<T> A, int mode, double host_dt,
+                                                   int host_clamp) {
+  const int chain = A.chain0 + blockIdx.x;
+  const ChainParams &P = A.prm[chain];
+  if (!P.active) return;
+  ChainState &st = A.st[chain];
+  const int lane = threadIdx.x;
+  if (mode != 2) {
+    const int n = A.Nt * A.ntiles;
+    const double *pp = A.partials + int64_t(chain) * n * 2;
+    double m = 0.0, s = 0.0;
+    for (int i = lane; i < n; i += 64) {
+      m = fmax(m, pp[2 * i]);
+      s += pp[2 * i + 1];
+    }
+    m = wave_max(m);
+    s = wave_sum(s);
+    if (lane == 0) {
+      const double mean = s / double(4 * A.S);
+      st.max_drift = m;
+      st.mean_drift = mean;
+      if (!isfinite(m) || !isfinite(s)) st.nonfinite = 1;
+      if (mode == 0) {
+        int clamp = 0;
+        const double dt = new_dt(mean, m, P, &clamp);
+        st.dt = dt;
+        st.cur_step = st.steps;
+        st.steps += 1;
+        st.t_langevin += dt;
+        if (clamp < 0) st.clamp_low += 1;
+        if (clamp > 0) st.clamp_high += 1;
+        st.in_parity = st.parity;  // this step reads the current buffer
+        st.parity = 1 - st.parity; // and its output becomes current
+      }
+    }
+  } else if (lane == 0) {
+    st.dt = host_dt;
+    st.cur_step = st.steps;
+    st.steps += 1;
+    st.t_langevin += host_dt;
+    if (host_clamp < 0) st.clamp_low += 1;
+    if (host_clamp > 0) st.clamp_high += 1;
+    st.in_parity = st.parity;
+    st.parity = 1 - st.parity;
+  }
+}
+
+// The update of one tau-slice of `rows` rows: multiply_by_timestep + add_noise
+// + fields += drift (Propagator_without_cutoff.cu:131-157, 432-441) with the
+// drift recomputed in registers and the noise drawn from Philox in registers.
+template <typename T, int X>
+__global__ __launch_bounds__(256) void k_update(BucketArgs<T> A) {
+  extern __shared__ __align__(16) char smem_raw[];
+  const int chain = A.chain0 + blockIdx.z;
+  const ChainParams &P = A.prm[chain];
+  if (!P.active) return;
+  const int tile = blockIdx.x, t = blockIdx.y;
+  const int tid = threadIdx.x, nthr = blockDim.x;
+  const int j = tid / X, x = tid - (tid / X) * X;
+  const bool has_site = j < A.rows;
+  Smem<T, X> sm(smem_raw, 4 * A.rows);
+  load_tables<T, X>(sm, A.tw, P.a_x, tid, nthr);
+  const ChainState &st = A.st[chain];
+  const int par = st.in_parity;
+  const double dt = st.dt;
+  const uint64_t step = uint64_t(st.cur_step);
+  const C2<T> *psi = (par ? A.buf1_psi : A.buf0_psi) + chain * A.chain_stride;
+  const C2<T> *psib = (par ? A.buf1_psib : A.buf0_psib) + chain * A.chain_stride;
+  C2<T> *opsi = (par ? A.buf0_psi : A.buf1_psi) + chain * A.chain_stride;
+  C2<T> *opsib = (par ? A.buf0_psib : A.buf1_psib) + chain * A.chain_stride;
+  const int r = tile * A.rows + j;
+  const int tm1 = (t == 0) ? A.Nt - 1 : t - 1;
+  const int tp1 = (t == A.Nt - 1) ? 0 : t + 1;
+  C2<T> pm[4], p0[4], pp[4], b0[4], bp[4], km[4], kp[4];
+  const int64_t s0 = (int64_t(t) * A.R + r) * X + x;
+  if (has_site) {
+    load4(psi, (int64_t(tm1) * A.R + r) * X + x, pm);
+    load4(psi, s0, p0);
+    load4(psi, (int64_t(tp1) * A.R + r) * X + x, pp);
+    load4(psib, s0, b0);
+    load4(psib, (int64_t(tp1) * A.R + r) * X + x, bp);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) sm.a[(c * A.rows + j) * X + x] = pm[c];
+  }
+  __syncthreads();
+  const C2<T> *K = kinetic_lines<T, X>(sm.a, sm.b, sm.tw, sm.mult, 4 * A.rows, tid, nthr);
+  if (has_site) {
+#pragma unroll
+    for (int c = 0; c < 4; ++c) km[c] = K[(c * A.rows + j) * X + x];
+  }
+  __syncthreads();
+  if (has_site) {
+#pragma unroll
+    for (int c = 0; c < 4; ++c) sm.a[(c * A.rows + j) * X + x] = bp[c];
+  }
+  __syncthreads();
+  K = kinetic_lines<T, X>(sm.a, sm.b, sm.tw, sm.mult, 4 * A.rows, tid, nthr);
+  if (!has_site) return;
+#pragma unroll
+  for (int c = 0; c < 4; ++c) kp[c] = K[(c * A.rows + j) * X + x];
+  const Coef<T> k = coef<T>(P);
+  C2<T> D[4], Db[4];
+  drift_site(k, pm, p0, pp, b0, bp, km, kp, D, Db);
+  T nz[8] = {};
+  if (P.noise_on) {
+    const uint32_t site = uint32_t((int64_t(t) * X + x) * A.R + r);  // reference site order
+    normals4(nz, P.seed, 0u, step, site, 0u);
+    normals4(nz + 4, P.seed, 0u, step, site, 1u);
+    const T sq = T(sqrt(dt));
+#pragma unroll
+    for (int q = 0; q < 8; ++q) nz[q] *= sq;
+  }
+  const T dtT = T(dt);
+  C2<T> o[4], ob[4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    const C2<T> eta = {nz[2 * c], nz[2 * c + 1]};
+    o[c] = cadd(p0[c], cadd(cscale(dtT, D[c]), eta));
+    ob[c] = cadd(b0[c], cadd(cscale(dtT, Db[c]), cconj(eta)));
+  }
+  store4(opsi, s0, o);
+  store4(opsib, s0, ob);
+}
+
+// Observables of the current fields over one tau-chunk: n_c (real space,
+// equal to the k-space sums of evaluate_n_* by Parseval), F_a(x)
+// (evaluate_spin_density_realspace, Observables_without_cutoff.cu:456-488)
+// and the energy density h on the time-split pair (psibar(t+1), psi(t)).
+template <typename T, int X>
+__global__ __launch_bounds__(256) void k_obs(BucketArgs<T> A) {
+  extern __shared__ __align__(16) char smem_raw[];
+  const int chain = A.chain0 + blockIdx.z;
+  const ChainParams &P = A.prm[chain];
+  if (!P.active) return;
+  const ChainState &st = A.st[chain];
+  if (A.eval_every > 0 && (st.cur_step % A.eval_every) != 0) return;
+  const int tile = blockIdx.x, chunk = blockIdx.y;
+  const int tid = threadIdx.x, nthr = blockDim.x;
+  const int j = tid / X, x = tid - (tid / X) * X;
+  const bool has_site = j < A.rows;
+  Smem<T, X> sm(smem_raw, 4 * A.rows);
+  load_tables<T, X>(sm, A.tw, P.a_x, tid, nthr);
+  const int par = st.parity;
+  const C2<T> *psi = (par ? A.buf1_psi : A.buf0_psi) + chain * A.chain_stride;
+  const C2<T> *psib = (par ? A.buf1_psib : A.buf0_psib) + chain * A.chain_stride;
+  const int r = tile * A.rows + j;
+  const double ax = P.a_x, G = P.G, c1 = P.c1;
+  const double sz[4] = {P.P + P.Q, 0.0, 0.0, -P.P + P.Q};
+  double n[4] = {0, 0, 0, 0}, e = 0.0, F[3] = {0, 0, 0};
+  for (int tt = 0; tt < A.tc; ++tt) {
+    const int t = chunk * A.tc + tt;
+    const int tp1 = (t == A.Nt - 1) ? 0 : t + 1;
+    C2<T> q[4], b[4];
+    if (has_site) {
+      load4(psi, (int64_t(t) * A.R + r) * X + x, q);
+      load4(psib, (int64_t(tp1) * A.R + r) * X + x, b);
+#pragma unroll
+      for (int c = 0; c < 4; ++c) sm.a[(c * A.rows + j) * X + x] = q[c];
+    }
+    __syncthreads();
+    const C2<T> *K = kinetic_lines<T, X>(sm.a, sm.b, sm.tw, sm.mult, 4 * A.rows, tid, nthr);
+    if (has_site) {
+      C2<double> bb[4], qq[4], kk[4];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const C2<T> kc = K[(c * A.rows + j) * X + x];
+        bb[c] = {double(b[c].x), double(b[c].y)};
+        qq[c] = {double(q[c].x), double(q[c].y)};
+        kk[c] = {double(kc.x), double(kc.y)};
+      }
+      C2<double> h = {0.0, 0.0}, nn = {0.0, 0.0};
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const C2<double> bq = cmul(bb[c], qq[c]);
+        n[c] += bq.x * ax;
+        nn = cadd(nn, bq);
+        h = csub(h, cadd(cmul(bb[c], kk[c]), cscale(sz[c], bq)));
+      }
+      const C2<double> detb = csub(cmul(bb[0], bb[3]), cmul(bb[1], bb[2]));
+      const C2<double> detq = csub(cmul(qq[0], qq[3]), cmul(qq[1], qq[2]));
+      h = cadd(h, cscale(G, csub(cmul(nn, nn), cscale(2.0, cmul(detb, detq)))));
+      const C2<double> Fx = cadd(cadd(cmul(bb[2], qq[0]), cmul(bb[0], qq[2])),
+                                 cadd(cmul(bb[3], qq[1]), cmul(bb[1], qq[3])));
+      const C2<double> Fyi = csub(cadd(cmul(bb[2], qq[0]), cmul(bb[3], qq[1])),
+                                  cadd(cmul(bb[0], qq[2]), cmul(bb[1], qq[3])));
+      const C2<double> Fy = {-Fyi.y, Fyi.x};  // i * Fyi
+      const C2<double> Fz = csub(cadd(cmul(bb[0], qq[0]), cmul(bb[1], qq[1])),
+                                 cadd(cmul(bb[2], qq[2]), cmul(bb[3], qq[3])));
+      const C2<double> F2 = cadd(cadd(cmul(Fx, Fx), cmul(Fy, Fy)), cmul(Fz, Fz));
+      h = cadd(h, cscale(0.5 * c1, F2));
+      e += h.x * ax;
+      F[0] += Fx.x;
+      F[1] += Fy.x;
+      F[2] += Fz.x;
+    }
+    __syncthreads();
+  }
+  if (has_site) {
+    const int64_t vs = int64_t(x) * A.R + r;  // reference spatial order (x, y, z)
+    double *o = A.obs_stage + ((int64_t(chain) * A.nchunks + chunk) * A.Vs + vs) * 3;
+    o[0] = F[0];
+    o[1] = F[1];
+    o[2] = F[2];
+  }
+  double vals[5] = {n[0], n[1], n[2], n[3], e};
+  const int w = tid >> 6, nw = (nthr + 63) >> 6;
+#pragma unroll
+  for (int q = 0; q < 5; ++q) {
+    const double v = wave_sum(vals[q]);
+    if ((tid & 63) == 0) sm.red[q * 8 + w] = v;
+  }
+  __syncthreads();
+  if (tid < 5) {
+    double s = 0.0;
+    for (int q = 0; q < nw; ++q) s += sm.red[tid * 8 + q];
+    A.obs_spart[((int64_t(chain) * A.nchunks + chunk) * A.ntiles + tile) * 5 + tid] = s;
+  }
+}
+
+// Fixed-order accumulation of the chunk results into the running sums
+// (the accumulators of Observables::evaluate, :1274-1306, 1322-1382).
+template <typename T>
+__global__ __launch_bounds__(256) void k_obs_finish(BucketArgs<T> A) {
+  const int chain = A.chain0 + blockIdx.y;
+  const ChainParams &P = A.prm[chain];
+  if (!P.active) return;
+  const ChainState &st = A.st[chain];
+  if (A.eval_every > 0 && (st.cur_step % A.eval_every) != 0) return;
+  const int64_t vs = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (vs < A.Vs) {
+    for (int a = 0; a < 3; ++a) {
+      double s = 0.0;
+      for (int c = 0; c < A.nchunks; ++c)
+        s += A.obs_stage[((int64_t(chain) * A.nchunks + c) * A.Vs + vs) * 3 + a];
+      A.obs_F[(int64_t(chain) * A.Vs + vs) * 3 + a] += s;
+    }
+  }
+  if (blockIdx.x == 0 && threadIdx.x < 64) {
+    const int lane = threadIdx.x;
+    const int n = A.nchunks * A.ntiles;
+    const double *sp = A.obs_spart + int64_t(chain) * n * 5;
+    for (int q = 0; q < 5; ++q) {
+      double s = 0.0;
+      for (int i = lane; i < n; i += 64) s += sp[i * 5 + q];
+      s = wave_sum(s);
+      if (lane == 0) A.obs_acc[chain * 5 + q] += s;
+    }
+    if (lane == 0) A.obs_count[chain] += 1;
+  }
+}
+
+// psi_c = sqrt(n0) + sqrt(var) (g_re + i g_im), psibar = conj(psi); g from
+// the chain's Philox key, stream 1, step 0, at the reference site index.
+template <typename T>
+__global__ __launch_bounds__(256) void k_init(BucketArgs<T> A, double n0, double var) {
+  const int chain = A.chain0 + blockIdx.y;
+  const ChainParams &P = A.prm[chain];
+  const int64_t s = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (s >= A.S) return;
+  const int par = A.st[chain].parity;
+  C2<T> *psi = (par ? A.buf1_psi : A.buf0_psi) + chain * A.chain_stride;
+  C2<T> *psib = (par ? A.buf1_psib : A.buf0_psib) + chain * A.chain_stride;
+  // internal site s = (t*R + r)*X + x
+  const int64_t X = A.S / (int64_t(A.Nt) * A.R);
+  const int64_t x = s % X, tr = s / X;
+  const int64_t r = tr % A.R, t = tr / A.R;
+  const uint32_t site = uint32_t((t * X + x) * A.R + r);
+  double g[8];
+  normals4(g, P.seed, 1u, 0ull, site, 0u);
+  normals4(g + 4, P.seed, 1u, 0ull, site, 1u);
+  const double a = sqrt(n0), sd = sqrt(var);
+  C2<T> v[4], vb[4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    v[c] = {T(a + sd * g[2 * c]), T(sd * g[2 * c + 1])};
+    vb[c] = cconj(v[c]);
+  }
+  store4(psi, s, v);
+  store4(psib, s, vb);
+}
+
+// ========================================================== host side =====
+namespace {
+
+const char *kVersion = "clv 0.1 (gfx950)";
+thread_local std::string g_create_error = "";
+
+bool supported_x(int X) {
+  switch (X) {
+    case 4: case 8: case 16: case 24: case 32: case 48: case 64: case 96: case 128: case 192:
+    case 256:
+      return true;
+    default:
+      return false;
+  }
+}
+
+struct Bucket {
+  clv_shape shape;
+  int X = 0, R = 0, Nt = 0;
+  int64_t S = 0, Vs = 0;
+  int nch = 0;
+  std::vector<int> ids;  // global chain ids
+  int rows = 1, ntiles = 1, nthr = 64, tc = 1, nchunks = 1;
+  size_t esz = 0;        // bytes per complex element
+  void *psi[2] = {nullptr, nullptr}, *psib[2] = {nullptr, nullptr};
+  ChainParams *d_prm = nullptr;
+  ChainState *d_st = nullptr;
+  void *d_tw = nullptr;
+  double *d_part = nullptr, *d_stage = nullptr, *d_spart = nullptr, *d_F = nullptr,
+         *d_acc = nullptr;
+  int64_t *d_count = nullptr;
+  std::vector<int64_t> host_steps;
+  int schedule = 1;
+};
+
+}  // namespace
+
+struct clv_ctx {
+  int device = 0;
+  int precision = CLV_FP64;
+  hipStream_t stream = nullptr;
+  std::vector<Bucket> buckets;
+  std::vector<std::pair<int, int>> where;  // chain -> (bucket, local)
+  std::vector<clv_params> params;
+  std::vector<void *> allocs;
+  int64_t bytes = 0;
+  std::string err;
+  // kernel timing with HIP events on `stream`
+  bool profiling = false;
+  struct Pending { int kind; hipEvent_t a, b; };
+  std::vector<Pending> pending;
+  std::vector<hipEvent_t> event_pool;
+  double kt_ms[6] = {0, 0, 0, 0, 0, 0};
+  int64_t kt_n[6] = {0, 0, 0, 0, 0, 0};
+};
+
+namespace {
+
+int fail(clv_ctx *ctx, int code, const std::string &msg) {
+  if (ctx) ctx->err = msg;
+  else g_create_error = msg;
+  return code;
+}
+
+#define HIPCHK(ctx, expr)                                                              \
+  do {                                                                                 \
+    hipError_t e_ = (expr);                                                            \
+    if (e_ != hipSuccess)                                                              \
+      return fail(ctx, CLV_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+int dalloc(clv_ctx *ctx, void **p, size_t bytes) {
+  if (bytes == 0) bytes = 16;
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) {
+    *p = nullptr;
+    return fail(ctx, CLV_ERR_NOMEM, std::string("hipMalloc(") + std::to_string(bytes) +
+                                        "): " + hipGetErrorString(e));
+  }
+  ctx->allocs.push_back(*p);
+  ctx->bytes += int64_t(bytes);
+  return CLV_OK;
+}
+
+ChainParams to_device_params(const clv_params &p) {
+  ChainParams d;
+  d.mu = p.mu; d.P = p.P; d.G = p.G; d.c1 = p.c1; d.Q = p.Q; d.eps = p.eps; d.a_x = p.a_x;
+  d.DT = p.DT; d.dt_min = p.dt_min; d.dt_max = p.dt_max; d.alpha = p.alpha; d.factor = p.factor;
+  d.seed = p.seed; d.q3_mode = p.q3_mode; d.noise_on = p.noise_on; d.active = 1; d.pad = 0;
+  return d;
+}
+
+template <typename T>
+BucketArgs<T> make_args(const Bucket &b, int chain0, int eval_every) {
+  BucketArgs<T> A;
+  A.buf0_psi = static_cast<C2<T> *>(b.psi[0]);
+  A.buf0_psib = static_cast<C2<T> *>(b.psib[0]);
+  A.buf1_psi = static_cast<C2<T> *>(b.psi[1]);
+  A.buf1_psib = static_cast<C2<T> *>(b.psib[1]);
+  A.prm = b.d_prm;
+  A.st = b.d_st;
+  A.tw = static_cast<const C2<T> *>(b.d_tw);
+  A.partials = b.d_part;
+  A.obs_stage = b.d_stage;
+  A.obs_spart = b.d_spart;
+  A.obs_F = b.d_F;
+  A.obs_acc = b.d_acc;
+  A.obs_count = b.d_count;
+  A.chain_stride = 4 * b.S;
+  A.S = b.S;
+  A.Vs = b.Vs;
+  A.R = b.R;
+  A.Nt = b.Nt;
+  A.rows = b.rows;
+  A.ntiles = b.ntiles;
+  A.nchunks = b.nchunks;
+  A.tc = b.tc;
+  A.chain0 = chain0;
+  A.eval_every = eval_every;
+  return A;
+}
+
+// ---- launchers, dispatched on X at run time ----
+template <typename T, int X>
+void launch_row_kernels(const Bucket &b, hipStream_t s, int which, int chain0, int nch,
+                        int eval_every) {
+  BucketArgs<T> A = make_args<T>(b, chain0, eval_every);
+  const dim3 block(b.nthr);
+  if (which == 0) {
+    const size_t sh = Smem<T, X>::bytes(b.rows);
+    hipLaunchKernelGGL((k_drift_stats<T, X>), dim3(b.ntiles, b.Nt, nch), block, sh, s, A);
+  } else if (which == 1) {
+    const size_t sh = Smem<T, X>::bytes(4 * b.rows);
+    hipLaunchKernelGGL((k_update<T, X>), dim3(b.ntiles, b.Nt, nch), block, sh, s, A);
+  } else {
+    const size_t sh = Smem<T, X>::bytes(4 * b.rows);
+    hipLaunchKernelGGL((k_obs<T, X>), dim3(b.ntiles, b.nchunks, nch), block, sh, s, A);
+  }
+}
+
+template <typename T>
+void dispatch_row(const Bucket &b, hipStream_t s, int which, int chain0, int nch, int ev) {
+  switch (b.X) {
+    case 4: launch_row_kernels<T, 4>(b, s, which, chain0, nch, ev); break;
+    case 8: launch_row_kernels<T, 8>(b, s, which, chain0, nch, ev); break;
+    case 16: launch_row_kernels<T, 16>(b, s, which, chain0, nch, ev); break;
+    case 24: launch_row_kernels<T, 24>(b, s, which, chain0, nch, ev); break;
+    case 32: launch_row_kernels<T, 32>(b, s, which, chain0, nch, ev); break;
+    case 48: launch_row_kernels<T, 48>(b, s, which, chain0, nch, ev); break;
+    case 64: launch_row_kernels<T, 64>(b, s, which, chain0, nch, ev); break;
+    case 96: launch_row_kernels<T, 96>(b, s, which, chain0, nch, ev); break;
+    case 128: launch_row_kernels<T, 128>(b, s, which, chain0, nch, ev); break;
+    case 192: launch_row_kernels<T, 192>(b, s, which, chain0, nch, ev); break;
+    case 256: launch_row_kernels<T, 256>(b, s, which, chain0, nch, ev); break;
+    default: break;
+  }
+}
+
+template <typename T>
+void launch_controller(const Bucket &b, hipStream_t s, int chain0, int nch, int mode, double dt,
+                       int clamp) {
+  BucketArgs<T> A = make_args<T>(b, chain0, 0);
+  hipLaunchKernelGGL((k_controller<T>), dim3(nch), dim3(64), 0, s, A, mode, dt, clamp);
+}
+
+template <typename T>
+void launch_obs_finish(const Bucket &b, hipStream_t s, int chain0, int nch, int ev) {
+  BucketArgs<T> A = make_args<T>(b, chain0, ev);
+  const int gx = int((b.Vs + 255) / 256);
+  hipLaunchKernelGGL((k_obs_finish<T>), dim3(gx, nch), dim3(256), 0, s, A);
+}
+
+template <typename T>
+void launch_init(const Bucket &b, hipStream_t s, int chain0, double n0, double var) {
+  BucketArgs<T> A = make_args<T>(b, chain0, 0);
+  const int gx = int((b.S + 255) / 256);
+  hipLaunchKernelGGL((k_init<T>), dim3(gx, 1), dim3(256), 0, s, A, n0, var);
+}
+
+hipEvent_t take_event(clv_ctx *c) {
+  if (!c->event_pool.empty()) {
+    hipEvent_t e = c->event_pool.back();
+    c->event_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+// Launch `fn` on the context's stream; with profiling on, bracket it with
+// events of kind `kind`.
+template <class F>
+void timed(clv_ctx *c, int kind, F &&fn) {
+  if (!c->profiling) {
+    fn();
+    return;
+  }
+  hipEvent_t a = take_event(c), b = take_event(c);
+  if (a) (void)hipEventRecord(a, c->stream);
+  fn();
+  if (b) (void)hipEventRecord(b, c->stream);
+  if (a && b) c->pending.push_back({kind, a, b});
+}
+
+void drain_events(clv_ctx *c) {
+  for (auto &p : c->pending) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+      c->kt_ms[p.kind] += ms;
+      c->kt_n[p.kind] += 1;
+    }
+    c->event_pool.push_back(p.a);
+    c->event_pool.push_back(p.b);
+  }
+  c->pending.clear();
+}
+
+// precision-dispatching wrappers
+void row_kernel(clv_ctx *c, const Bucket &b, int which, int chain0, int nch, int ev) {
+  const int kind = which == 0 ? 0 : (which == 1 ? 2 : 3);
+  timed(c, kind, [&] {
+    if (c->precision == CLV_FP32) dispatch_row<float>(b, c->stream, which, chain0, nch, ev);
+    else dispatch_row<double>(b, c->stream, which, chain0, nch, ev);
+  });
+}
+void controller(clv_ctx *c, const Bucket &b, int chain0, int nch, int mode, double dt, int cl) {
+  timed(c, 1, [&] {
+    if (c->precision == CLV_FP32) launch_controller<float>(b, c->stream, chain0, nch, mode, dt, cl);
+    else launch_controller<double>(b, c->stream, chain0, nch, mode, dt, cl);
+  });
+}
+void obs_finish(clv_ctx *c, const Bucket &b, int chain0, int nch, int ev) {
+  timed(c, 4, [&] {
+    if (c->precision == CLV_FP32) launch_obs_finish<float>(b, c->stream, chain0, nch, ev);
+    else launch_obs_finish<double>(b, c->stream, chain0, nch, ev);
+  });
+}
+
+// rows per workgroup: the largest divisor of R with X*rows <= 256
+int pick_rows(int X, int R) {
+  int best = 1;
+  for (int q = 1; q <= R && q * X <= 256; ++q)
+    if (R % q == 0) best = q;
+  return best;
+}
+
+int setup_bucket(clv_ctx *ctx, Bucket &b) {
+  const clv_shape &sh = b.shape;
+  b.X = sh.nx;
+  b.R = sh.ny * sh.nz;
+  b.Nt = sh.nt;
+  b.Vs = int64_t(sh.nx) * sh.ny * sh.nz;
+  b.S = b.Vs * sh.nt;
+  b.esz = (ctx->precision == CLV_FP32) ? 8 : 16;
+  b.rows = pick_rows(b.X, b.R);
+  b.ntiles = b.R / b.rows;
+  b.nthr = ((b.X * b.rows + 63) / 64) * 64;
+  b.tc = 1;
+  for (int q = 1; q <= 8; ++q)
+    if (b.Nt % q == 0) b.tc = q;
+  b.nchunks = b.Nt / b.tc;
+  const size_t field = size_t(b.nch) * size_t(b.S) * 4 * b.esz;
+  int rc;
+  for (int k = 0; k < 2; ++k) {
+    if ((rc = dalloc(ctx, &b.psi[k], field))) return rc;
+    if ((rc = dalloc(ctx, &b.psib[k], field))) return rc;
+  }
+  if ((rc = dalloc(ctx, reinterpret_cast<void **>(&b.d_prm), sizeof(ChainParams) * b.nch))) return rc;
+  if ((rc = dalloc(ctx, reinterpret_cast<void **>(&b.d_st), sizeof(ChainState) * b.nch))) return rc;
+  if ((rc = dalloc(ctx, &b.d_tw, b.esz * b.X))) return rc;
+  if ((rc = dalloc(ctx, reinterpret_cast<void **>(&b.d_part),
+                   sizeof(double) * 2 * b.nch * b.Nt * b.ntiles)))
+    return rc;
+  if ((rc = dalloc(ctx, reinterpret_cast<void **>(&b.d_stage),
+                   sizeof(double) * 3 * b.nch * b.nchunks * b.Vs)))
+    return rc;
+  if ((rc = dalloc(ctx, reinterpret_cast<void **>(&b.d_spart),
+                   sizeof(double) * 5 * b.nch * b.nchunks * b.ntiles)))
+    return rc;
+  if ((rc = dalloc(ctx, reinterpret_cast<void **>(&b.d_F), sizeof(double) * 3 * b.nch * b.Vs))) return rc;
+  if ((rc = dalloc(ctx, reinterpret_cast<void **>(&b.d_acc), sizeof(double) * 5 * b.nch))) return rc;
+  if ((rc = dalloc(ctx, reinterpret_cast<void **>(&b.d_count), sizeof(int64_t) * b.nch))) return rc;
+  // twiddles exp(-2 pi i m / X), computed in FP64 on the host
+  std::vector<double> tw(2 * b.X);
+  for (int m = 0; m < b.X; ++m) {
+    const double a = 6.283185307179586 * double(m) / double(b.X);
+    tw[2 * m] = std::cos(a);
+    tw[2 * m + 1] = -std::sin(a);
+  }
+  if (ctx->precision == CLV_FP32) {
+    std::vector<float> twf(tw.begin(), tw.end());
+    HIPCHK(ctx, hipMemcpy(b.d_tw, twf.data(), sizeof(float) * twf.size(), hipMemcpyHostToDevice));
+  } else {
+    HIPCHK(ctx, hipMemcpy(b.d_tw, tw.data(), sizeof(double) * tw.size(), hipMemcpyHostToDevice));
+  }
+  HIPCHK(ctx, hipMemset(b.d_st, 0, sizeof(ChainState) * b.nch));
+  HIPCHK(ctx, hipMemset(b.d_F, 0, sizeof(double) * 3 * b.nch * b.Vs));
+  HIPCHK(ctx, hipMemset(b.d_acc, 0, sizeof(double) * 5 * b.nch));
+  HIPCHK(ctx, hipMemset(b.d_count, 0, sizeof(int64_t) * b.nch));
+  for (int k = 0; k < 2; ++k) {
+    HIPCHK(ctx, hipMemset(b.psi[k], 0, field));
+    HIPCHK(ctx, hipMemset(b.psib[k], 0, field));
+  }
+  b.host_steps.assign(b.nch, 0);
+  return CLV_OK;
+}
+
+int check_chain(clv_ctx *ctx, int chain) {
+  if (!ctx) return CLV_ERR_ARG;
+  if (chain < 0 || chain >= int(ctx->where.size()))
+    return fail(ctx, CLV_ERR_NOTFOUND, "chain index " + std::to_string(chain) + " out of range");
+  return CLV_OK;
+}
+
+int read_state(clv_ctx *ctx, const Bucket &b, int local, ChainState *out) {
+  HIPCHK(ctx, hipMemcpyAsync(out, b.d_st + local, sizeof(ChainState), hipMemcpyDeviceToHost,
+                             ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return CLV_OK;
+}
+
+// Does any chain of the bucket evaluate after its next step?
+bool bucket_needs_eval(const Bucket &b, int ev) {
+  if (ev <= 0) return false;
+  for (int64_t s : b.host_steps)
+    if (s % ev == 0) return true;
+  return false;
+}
+
+}  // namespace
+
+// =============================================================== C ABI ====
+extern "C" {
+
+const char *clv_version(void) { return kVersion; }
+
+const char *clv_last_error(const clv_ctx *ctx) {
+  if (ctx) return ctx->err.c_str();
+  return g_create_error.c_str();
+}
+
+void clv_default_params(clv_params *p) {
+  std::memset(p, 0, sizeof(*p));
+  p->mu = 0.0;
+  p->P = 0.0;
+  p->G = 0.1;      // Parameters.h:15
+  p->c1 = 0.0;
+  p->Q = 0.0;      // Parameters.h:24
+  p->eps = 0.01;   // Parameters.h:13
+  p->a_x = 1.0;    // Parameters.h:14
+  p->DT = 0.002;   // Parameters.h:29
+  p->dt_min = 1e-8;
+  p->dt_max = 1e-2;
+  p->alpha = 0.9;
+  p->factor = 1.0;
+  p->seed = 0;
+  p->kin_dims = 1;
+  p->q3_mode = CLV_Q3_REFERENCE;
+  p->noise_on = 1;
+}
+
+double clv_compute_new_dt(double mean_drift, double max_drift, const clv_params *p,
+                          int32_t *clamp) {
+  double dt = (max_drift <= 0.0) ? p->DT : (mean_drift / max_drift) * p->DT;
+  int32_t c = 0;
+  if (dt < p->dt_min) { dt = p->dt_min; c = -1; }
+  if (dt > p->dt_max) { dt = p->dt_max; c = 1; }
+  if (clamp) *clamp = c;
+  return (1 - p->alpha) * p->DT + p->alpha * dt;
+}
+
+int clv_create(const clv_shape *shapes, int32_t nchains, int32_t precision, int32_t device,
+               clv_ctx **out) {
+  if (!out || !shapes || nchains <= 0) return fail(nullptr, CLV_ERR_ARG, "clv_create: bad arguments");
+  *out = nullptr;
+  if (precision != CLV_FP32 && precision != CLV_FP64)
+    return fail(nullptr, CLV_ERR_ARG, "clv_create: precision must be CLV_FP32 or CLV_FP64");
+  for (int i = 0; i < nchains; ++i) {
+    const clv_shape &s = shapes[i];
+    if (s.nx <= 0 || s.ny <= 0 || s.nz <= 0 || s.nt < 2)
+      return fail(nullptr, CLV_ERR_ARG, "clv_create: chain " + std::to_string(i) + " has a bad shape");
+    if (!supported_x(s.nx))
+      return fail(nullptr, CLV_ERR_ARG,
+                  "clv_create: X=" + std::to_string(s.nx) +
+                      " not supported (row transform lengths 4,8,16,24,32,48,64,96,128,192,256)");
+    if (int64_t(s.nx) * s.ny * s.nz * s.nt >= (int64_t(1) << 31))
+      return fail(nullptr, CLV_ERR_ARG, "clv_create: lattice too large");
+  }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+    return fail(nullptr, CLV_ERR_HIP, "clv_create: no HIP device visible");
+  if (device < 0 || device >= ndev) return fail(nullptr, CLV_ERR_ARG, "clv_create: bad device index");
+  std::unique_ptr<clv_ctx> ctx(new clv_ctx());
+  ctx->device = device;
+  ctx->precision = precision;
+  if (hipSetDevice(device) != hipSuccess) return fail(nullptr, CLV_ERR_HIP, "hipSetDevice failed");
+  if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess)
+    return fail(nullptr, CLV_ERR_HIP, "hipStreamCreate failed");
+  // group chains into buckets of equal shape
+  std::map<std::tuple<int, int, int, int>, int> key2b;
+  ctx->where.resize(nchains);
+  for (int i = 0; i < nchains; ++i) {
+    const clv_shape &s = shapes[i];
+    auto key = std::make_tuple(s.nx, s.ny, s.nz, s.nt);
+    auto it = key2b.find(key);
+    int bi;
+    if (it == key2b.end()) {
+      bi = int(ctx->buckets.size());
+      key2b[key] = bi;
+      ctx->buckets.emplace_back();
+      ctx->buckets.back().shape = s;
+    } else {
+      bi = it->second;
+    }
+    Bucket &b = ctx->buckets[bi];
+    ctx->where[i] = {bi, b.nch};
+    b.ids.push_back(i);
+    b.nch += 1;
+  }
+  for (Bucket &b : ctx->buckets) {
+    int rc = setup_bucket(ctx.get(), b);
+    if (rc) {
+      g_create_error = ctx->err;
+      clv_destroy(ctx.release());
+      return rc;
+    }
+  }
+  ctx->params.resize(nchains);
+  for (int i = 0; i < nchains; ++i) {
+    clv_default_params(&ctx->params[i]);
+    ctx->params[i].seed = uint64_t(i);
+    int rc = clv_set_params(ctx.get(), i, &ctx->params[i]);
+    if (rc) {
+      g_create_error = ctx->err;
+      clv_destroy(ctx.release());
+      return rc;
+    }
+  }
+  *out = ctx.release();
+  return CLV_OK;
+}
+
+int clv_destroy(clv_ctx *ctx) {
+  if (!ctx) return CLV_OK;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  drain_events(ctx);
+  for (hipEvent_t e : ctx->event_pool) (void)hipEventDestroy(e);
+  for (void *p : ctx->allocs) (void)hipFree(p);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return CLV_OK;
+}
+
+int clv_num_chains(const clv_ctx *ctx) { return ctx ? int(ctx->where.size()) : 0; }
+int64_t clv_device_bytes(const clv_ctx *ctx) { return ctx ? ctx->bytes : 0; }
+
+int clv_set_params(clv_ctx *ctx, int32_t chain, const clv_params *p) {
+  int rc = check_chain(ctx, chain);
+  if (rc) return rc;
+  if (!p) return fail(ctx, CLV_ERR_ARG, "clv_set_params: NULL params");
+  if (p->kin_dims != 1)
+    return fail(ctx, CLV_ERR_ARG,
+                "clv_set_params: kin_dims must be 1 (p^2 = p_x^2, the reference's kinetic term); "
+                "the full Laplacian is not built on the GPU path");
+  if (p->q3_mode != CLV_Q3_REFERENCE && p->q3_mode != CLV_Q3_PSIBAR)
+    return fail(ctx, CLV_ERR_ARG, "clv_set_params: bad q3_mode");
+  if (!(p->a_x > 0.0) || !(p->DT > 0.0) || !(p->eps > 0.0))
+    return fail(ctx, CLV_ERR_ARG, "clv_set_params: a_x, DT and eps must be positive");
+  (void)hipSetDevice(ctx->device);
+  ctx->params[chain] = *p;
+  const auto [bi, local] = ctx->where[chain];
+  Bucket &b = ctx->buckets[bi];
+  ChainParams d = to_device_params(*p);
+  HIPCHK(ctx, hipMemcpyAsync(b.d_prm + local, &d, sizeof(d), hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return CLV_OK;
+}
+
+int clv_set_fields(clv_ctx *ctx, int32_t chain, const double *psi, const double *psibar) {
+  int rc = check_chain(ctx, chain);
+  if (rc) return rc;
+  if (!psi || !psibar) return fail(ctx, CLV_ERR_ARG, "clv_set_fields: NULL field");
+  (void)hipSetDevice(ctx->device);
+  const auto [bi, local] = ctx->where[chain];
+  Bucket &b = ctx->buckets[bi];
+  ChainState st;
+  if ((rc = read_state(ctx, b, local, &st))) return rc;
+  const int par = st.parity;
+  const int X = b.X, R = b.R, Nt = b.Nt;
+  const size_t n = size_t(b.S) * 4;
+  for (int f = 0; f < 2; ++f) {
+    const double *src = f ? psibar : psi;
+    std::vector<char> host(n * b.esz);
+    for (int c = 0; c < 4; ++c)
+      for (int t = 0; t < Nt; ++t)
+        for (int x = 0; x < X; ++x)
+          for (int r = 0; r < R; ++r) {
+            const size_t ref = ((size_t(c) * Nt + t) * X + x) * R + r;
+            const size_t in = ((size_t(t) * R + r) * X + x) * 4 + c;
+            if (ctx->precision == CLV_FP32) {
+              float *d = reinterpret_cast<float *>(host.data());
+              d[2 * in] = float(src[2 * ref]);
+              d[2 * in + 1] = float(src[2 * ref + 1]);
+            } else {
+              double *d = reinterpret_cast<double *>(host.data());
+              d[2 * in] = src[2 * ref];
+              d[2 * in + 1] = src[2 * ref + 1];
+            }
+          }
+    void *dst = static_cast<char *>(f ? b.psib[par] : b.psi[par]) + size_t(local) * n * b.esz;
+    HIPCHK(ctx, hipMemcpy(dst, host.data(), n * b.esz, hipMemcpyHostToDevice));
+  }
+  return CLV_OK;
+}
+
+int clv_get_fields(clv_ctx *ctx, int32_t chain, double *psi, double *psibar) {
+  int rc = check_chain(ctx, chain);
+  if (rc) return rc;
+  (void)hipSetDevice(ctx->device);
+  const auto [bi, local] = ctx->where[chain];
+  Bucket &b = ctx->buckets[bi];
+  ChainState st;
+  if ((rc = read_state(ctx, b, local, &st))) return rc;
+  const int par = st.parity;
+  const int X = b.X, R = b.R, Nt = b.Nt;
+  const size_t n = size_t(b.S) * 4;
+  for (int f = 0; f < 2; ++f) {
+    double *dst = f ? psibar : psi;
+    if (!dst) continue;
+    std::vector<char> host(n * b.esz);
+    const void *src = static_cast<const char *>(f ? b.psib[par] : b.psi[par]) + size_t(local) * n * b.esz;
+    HIPCHK(ctx, hipMemcpy(host.data(), src, n * b.esz, hipMemcpyDeviceToHost));
+    for (int c = 0; c < 4; ++c)
+      for (int t = 0; t < Nt; ++t)
+        for (int x = 0; x < X; ++x)
+          for (int r = 0; r < R; ++r) {
+            const size_t ref = ((size_t(c) * Nt + t) * X + x) * R + r;
+            const size_t in = ((size_t(t) * R + r) * X + x) * 4 + c;
+            if (ctx->precision == CLV_FP32) {
+              const float *s = reinterpret_cast<const float *>(host.data());
+              dst[2 * ref] = s[2 * in];
+              dst[2 * ref + 1] = s[2 * in + 1];
+            } else {
+              const double *s = reinterpret_cast<const double *>(host.data());
+              dst[2 * ref] = s[2 * in];
+              dst[2 * ref + 1] = s[2 * in + 1];
+            }
+          }
+  }
+  return CLV_OK;
+}
+
+int clv_init_fields(clv_ctx *ctx, int32_t chain, double n0, double var) {
+  int rc = check_chain(ctx, chain);
+  if (rc) return rc;
+  if (n0 < 0 || var < 0) return fail(ctx, CLV_ERR_ARG, "clv_init_fields: n0 and var must be >= 0");
+  (void)hipSetDevice(ctx->device);
+  const auto [bi, local] = ctx->where[chain];
+  Bucket &b = ctx->buckets[bi];
+  if (ctx->precision == CLV_FP32) launch_init<float>(b, ctx->stream, local, n0, var);
+  else launch_init<double>(b, ctx->stream, local, n0, var);
+  HIPCHK(ctx, hipGetLastError());
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return CLV_OK;
+}
+
+static int step_bucket(clv_ctx *ctx, Bucket &b, int eval_every) {
+  row_kernel(ctx, b, 0, 0, b.nch, 0);
+  controller(ctx, b, 0, b.nch, 0, 0.0, 0);
+  row_kernel(ctx, b, 1, 0, b.nch, 0);
+  const bool ev = bucket_needs_eval(b, eval_every);
+  for (auto &s : b.host_steps) s += 1;
+  if (ev) {
+    row_kernel(ctx, b, 2, 0, b.nch, eval_every);
+    obs_finish(ctx, b, 0, b.nch, eval_every);
+  }
+  return CLV_OK;
+}
+
+int clv_propagate(clv_ctx *ctx, int64_t nsteps, int32_t eval_every) {
+  if (!ctx) return CLV_ERR_ARG;
+  if (nsteps < 0) return fail(ctx, CLV_ERR_ARG, "clv_propagate: nsteps < 0");
+  (void)hipSetDevice(ctx->device);
+  for (int64_t n = 0; n < nsteps; ++n)
+    for (Bucket &b : ctx->buckets) step_bucket(ctx, b, eval_every);
+  HIPCHK(ctx, hipGetLastError());
+  return CLV_OK;
+}
+
+int clv_drift_stats(clv_ctx *ctx, int32_t chain, double *max_drift, double *mean_drift) {
+  int rc = check_chain(ctx, chain);
+  if (rc) return rc;
+  (void)hipSetDevice(ctx->device);
+  const auto [bi, local] = ctx->where[chain];
+  Bucket &b = ctx->buckets[bi];
+  row_kernel(ctx, b, 0, local, 1, 0);
+  controller(ctx, b, local, 1, 1, 0.0, 0);
+  HIPCHK(ctx, hipGetLastError());
+  ChainState st;
+  if ((rc = read_state(ctx, b, local, &st))) return rc;
+  if (max_drift) *max_drift = st.max_drift;
+  if (mean_drift) *mean_drift = st.mean_drift;
+  return CLV_OK;
+}
+
+int clv_apply_update(clv_ctx *ctx, int32_t chain, double dt) {
+  int rc = check_chain(ctx, chain);
+  if (rc) return rc;
+  if (!(dt >= 0.0)) return fail(ctx, CLV_ERR_ARG, "clv_apply_update: dt must be >= 0");
+  (void)hipSetDevice(ctx->device);
+  const auto [bi, local] = ctx->where[chain];
+  Bucket &b = ctx->buckets[bi];
+  controller(ctx, b, local, 1, 2, dt, 0);
+  row_kernel(ctx, b, 1, local, 1, 0);
+  b.host_steps[local] += 1;
+  HIPCHK(ctx, hipGetLastError());
+  return CLV_OK;
+}
+
+int clv_propagate_plain(clv_ctx *ctx, int32_t chain, int64_t nsteps, int32_t eval_every) {
+  int rc = check_chain(ctx, chain);
+  if (rc) return rc;
+  (void)hipSetDevice(ctx->device);
+  const auto [bi, local] = ctx->where[chain];
+  Bucket &b = ctx->buckets[bi];
+  const clv_params &p = ctx->params[chain];
+  ChainState *pinned = nullptr;
+  HIPCHK(ctx, hipHostMalloc(reinterpret_cast<void **>(&pinned), sizeof(ChainState)));
+  for (int64_t n = 0; n < nsteps; ++n) {
+    // drift + max/mean on the device, blocking copy to the host
+    row_kernel(ctx, b, 0, local, 1, 0);
+    controller(ctx, b, local, 1, 1, 0.0, 0);
+    hipError_t e = hipMemcpyAsync(pinned, b.d_st + local, sizeof(ChainState),
+                                  hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) {
+      (void)hipHostFree(pinned);
+      return fail(ctx, CLV_ERR_HIP, std::string("plain step: ") + hipGetErrorString(e));
+    }
+    // dt on the host (compute_new_dt), passed by value to the update
+    int32_t clamp = 0;
+    const double dt = clv_compute_new_dt(pinned->mean_drift, pinned->max_drift, &p, &clamp);
+    controller(ctx, b, local, 1, 2, dt, clamp);
+    row_kernel(ctx, b, 1, local, 1, 0);
+    const bool ev = eval_every > 0 && (b.host_steps[local] % eval_every == 0);
+    b.host_steps[local] += 1;
+    if (ev) {
+      row_kernel(ctx, b, 2, local, 1, eval_every);
+      obs_finish(ctx, b, local, 1, eval_every);
+    }
+  }
+  (void)hipHostFree(pinned);
+  HIPCHK(ctx, hipGetLastError());
+  return CLV_OK;
+}
+
+int clv_get_step_stats(clv_ctx *ctx, int32_t chain, clv_step_stats *out) {
+  int rc = check_chain(ctx, chain);
+  if (rc) return rc;
+  if (!out) return fail(ctx, CLV_ERR_ARG, "clv_get_step_stats: NULL out");
+  (void)hipSetDevice(ctx->device);
+  const auto [bi, local] = ctx->where[chain];
+  ChainState st;
+  if ((rc = read_state(ctx, ctx->buckets[bi], local, &st))) return rc;
+  out->dt = st.dt;
+  out->max_drift = st.max_drift;
+  out->mean_drift = st.mean_drift;
+  out->t_langevin = st.t_langevin;
+  out->steps = st.steps;
+  out->clamp_low = st.clamp_low;
+  out->clamp_high = st.clamp_high;
+  out->nonfinite = st.nonfinite;
+  out->reserved = 0;
+  return CLV_OK;
+}
+
+int clv_synchronize(clv_ctx *ctx) {
+  if (!ctx) return CLV_ERR_ARG;
+  (void)hipSetDevice(ctx->device);
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return CLV_OK;
+}
+
+int clv_evaluate(clv_ctx *ctx, int32_t chain) {
+  if (!ctx) return CLV_ERR_ARG;
+  (void)hipSetDevice(ctx->device);
+  if (chain < 0) {
+    for (Bucket &b : ctx->buckets) {
+      row_kernel(ctx, b, 2, 0, b.nch, 0);
+      obs_finish(ctx, b, 0, b.nch, 0);
+    }
+  } else {
+    int rc = check_chain(ctx, chain);
+    if (rc) return rc;
+    const auto [bi, local] = ctx->where[chain];
+    Bucket &b = ctx->buckets[bi];
+    row_kernel(ctx, b, 2, local, 1, 0);
+    obs_finish(ctx, b, local, 1, 0);
+  }
+  HIPCHK(ctx, hipGetLastError());
+  return CLV_OK;
+}
+
+int clv_read_observables(clv_ctx *ctx, int32_t chain, clv_observables *out, double *Fx,
+                         double *Fy, double *Fz) {
+  int rc = check_chain(ctx, chain);
+  if (rc) return rc;
+  if (!out) return fail(ctx, CLV_ERR_ARG, "clv_read_observables: NULL out");
+  (void)hipSetDevice(ctx->device);
+  const auto [bi, local] = ctx->where[chain];
+  Bucket &b = ctx->buckets[bi];
+  double acc[5];
+  int64_t count = 0;
+  std::vector<double> F(3 * b.Vs);
+  HIPCHK(ctx, hipMemcpyAsync(acc, b.d_acc + 5 * local, sizeof(acc), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipMemcpyAsync(&count, b.d_count + local, sizeof(count), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipMemcpyAsync(F.data(), b.d_F + 3 * b.Vs * local, sizeof(double) * F.size(),
+                             hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  std::memset(out, 0, sizeof(*out));
+  out->count = count;
+  if (count == 0) return CLV_OK;
+  const double Nt = double(b.Nt), cnt = double(count), Vs = double(b.Vs);
+  double tot = 0.0;
+  for (int c = 0; c < 4; ++c) {
+    // real-space sums; the reference's k-space sums are V_s times larger and
+    // write_n_* divides by V_s (Observables_without_cutoff.cu:1527-1547)
+    out->n_comp[c] = acc[c] / Nt / cnt;
+    tot += acc[c];
+  }
+  out->n_tot = tot / Nt / cnt;
+  out->energy = acc[4] / Nt / cnt;
+  double *dst[3] = {Fx, Fy, Fz};
+  double tots[3] = {0, 0, 0};
+  for (int a = 0; a < 3; ++a)
+    for (int64_t s = 0; s < b.Vs; ++s) {
+      const double v = F[3 * s + a] / Nt / cnt;
+      tots[a] += v;
+      if (dst[a]) dst[a][s] = v;
+    }
+  out->Fx_tot = tots[0];
+  out->Fy_tot = tots[1];
+  out->Fz_tot = tots[2];
+  out->density = out->n_tot / Vs;
+  out->Sz = out->Fz_tot / Vs;
+  return CLV_OK;
+}
+
+int clv_reset_observables(clv_ctx *ctx, int32_t chain) {
+  if (!ctx) return CLV_ERR_ARG;
+  (void)hipSetDevice(ctx->device);
+  auto reset = [&](Bucket &b, int local, int n) -> int {
+    HIPCHK(ctx, hipMemsetAsync(b.d_F + 3 * b.Vs * local, 0, sizeof(double) * 3 * b.Vs * n, ctx->stream));
+    HIPCHK(ctx, hipMemsetAsync(b.d_acc + 5 * local, 0, sizeof(double) * 5 * n, ctx->stream));
+    HIPCHK(ctx, hipMemsetAsync(b.d_count + local, 0, sizeof(int64_t) * n, ctx->stream));
+    return CLV_OK;
+  };
+  int rc;
+  if (chain < 0) {
+    for (Bucket &b : ctx->buckets)
+      if ((rc = reset(b, 0, b.nch))) return rc;
+  } else {
+    if ((rc = check_chain(ctx, chain))) return rc;
+    const auto [bi, local] = ctx->where[chain];
+    if ((rc = reset(ctx->buckets[bi], local, 1))) return rc;
+  }
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return CLV_OK;
+}
+
+int clv_autotune(clv_ctx *ctx, const char *cache_path) {
+  (void)cache_path;
+  if (!ctx) return CLV_ERR_ARG;
+  return CLV_OK;
+}
+
+int clv_set_schedule(clv_ctx *ctx, int32_t schedule) {
+  if (!ctx) return CLV_ERR_ARG;
+  if (schedule < 0 || schedule > 2) return fail(ctx, CLV_ERR_ARG, "clv_set_schedule: bad schedule");
+  for (Bucket &b : ctx->buckets) b.schedule = schedule == 0 ? 1 : schedule;
+  return CLV_OK;
+}
+
+int clv_set_profiling(clv_ctx *ctx, int32_t on) {
+  if (!ctx) return CLV_ERR_ARG;
+  (void)hipSetDevice(ctx->device);
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  drain_events(ctx);
+  ctx->profiling = on != 0;
+  for (int k = 0; k < 6; ++k) {
+    ctx->kt_ms[k] = 0.0;
+    ctx->kt_n[k] = 0;
+  }
+  return CLV_OK;
+}
+
+int clv_kernel_time(clv_ctx *ctx, int32_t kind, double *total_ms, int64_t *launches) {
+  if (!ctx) return CLV_ERR_ARG;
+  if (kind < 0 || kind > 5) return fail(ctx, CLV_ERR_ARG, "clv_kernel_time: bad kind");
+  (void)hipSetDevice(ctx->device);
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  drain_events(ctx);
+  if (total_ms) *total_ms = ctx->kt_ms[kind];
+  if (launches) *launches = ctx->kt_n[kind];
+  return CLV_OK;
+}
+
+int clv_get_schedule(const clv_ctx *ctx, int32_t chain) {
+  if (!ctx || chain < 0 || chain >= int(ctx->where.size())) return CLV_ERR_ARG;
+  return ctx->buckets[ctx->where[chain].first].schedule;
+}
+
+}  // extern "C"

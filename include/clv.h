@@ -180,6 +180,15 @@ int clv_autotune(clv_ctx *ctx, const char *cache_path);
 int clv_set_schedule(clv_ctx *ctx, int32_t schedule);
 int clv_get_schedule(const clv_ctx *ctx, int32_t chain);
 
+/* ---- kernel timing -----------------------------------------------------
+ * With profiling on, every kernel the context launches is bracketed by HIP
+ * events on the context's stream; clv_kernel_time returns the accumulated
+ * device time (ms) and launch count of one kernel kind:
+ * 0 drift_stats, 1 controller, 2 update, 3 obs, 4 obs_finish, 5 fused step.
+ * (No counterpart in the reference, which times only the whole run, main.cu:59,159.) */
+int clv_set_profiling(clv_ctx *ctx, int32_t on);
+int clv_kernel_time(clv_ctx *ctx, int32_t kind, double *total_ms, int64_t *launches);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,215 @@
+"""Parity of the HIP engine (through the C ABI) with the FP64 CPU oracle.
+
+Tolerances (the bar the engine is held to):
+  FP64: per-site |engine - oracle| <= 1e-10 after 1000 steps on the same Philox
+        stream (configs[0] shape), <= 1e-12 for single-step quantities.
+  FP32: per-site |engine - oracle| <= 1e-4 * rms|oracle field| after 1000 steps.
+Quantities compared: drift statistics (max, mean of |D_0|), dt, Langevin time,
+fields after 1 and N steps (device controller and host-driven plain method),
+observables (n_c, F_a(x), energy).
+"""
+import numpy as np
+import pytest
+
+from cl_var_dt_amd import CLV_FP32, CLV_FP64, CLV_Q3_PSIBAR, Shape, default_params
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = [(64, 1, 1, 32), (16, 16, 1, 8), (8, 4, 4, 4), (24, 6, 1, 4), (128, 2, 1, 4),
+          (256, 1, 1, 4), (32, 8, 1, 16), (48, 3, 1, 6), (4, 2, 1, 8)]
+
+
+def _engine(shapes, precision=CLV_FP64):
+    from cl_var_dt_amd.engine import Engine
+    return Engine([Shape(*s) for s in shapes], precision=precision)
+
+
+def _params(seed=1, **kw):
+    base = dict(mu=1.0, G=0.1, c1=-0.05, Q=0.1, P=0.02, eps=1 / 32, DT=1e-3, seed=seed)
+    base.update(kw)
+    return default_params(**base)
+
+
+def _pair(oracle_lib, shape, precision=CLV_FP64, **kw):
+    p = _params(**kw)
+    eng = _engine([shape], precision)
+    eng.set_params(0, p)
+    orc = oracle_lib.Oracle(Shape(*shape), p, nthreads=4)
+    return eng, orc, p
+
+
+def test_fields_roundtrip_and_init(oracle_lib):
+    print("case: fields roundtrip", flush=True)
+    for shape in [(64, 1, 1, 32), (8, 4, 4, 4)]:
+        eng, orc, p = _pair(oracle_lib, shape)
+        rng = np.random.default_rng(0)
+        d = eng.dims(0)
+        f = rng.normal(size=d) + 1j * rng.normal(size=d)
+        g = rng.normal(size=d) + 1j * rng.normal(size=d)
+        eng.set_fields(0, f, g)
+        f2, g2 = eng.get_fields(0)
+        assert np.array_equal(f, f2) and np.array_equal(g, g2)
+        eng.init_fields(0, 1.0, 0.1)
+        orc.init_fields(1.0, 0.1)
+        a, b = eng.get_fields(0)
+        c, e = orc.get_fields()
+        assert np.max(abs(a - c)) < 1e-13 and np.array_equal(b, np.conj(a))
+        eng.close()
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+def test_drift_stats_and_one_step(oracle_lib, shape):
+    print("case: drift stats + one step", shape, flush=True)
+    eng, orc, p = _pair(oracle_lib, shape)
+    eng.init_fields(0, 1.0, 0.1)
+    orc.init_fields(1.0, 0.1)
+    mx_g, mean_g = eng.drift_stats(0)
+    mx_o, mean_o = orc.max_mean()
+    assert mx_g == pytest.approx(mx_o, rel=1e-12)
+    assert mean_g == pytest.approx(mean_o, rel=1e-12)
+    eng.propagate(1)
+    orc.propagate(1)
+    sg, so = eng.stats(0), orc.stats()
+    assert sg.steps == 1 and sg.dt == pytest.approx(so.dt, rel=1e-12)
+    a, b = eng.get_fields(0)
+    c, d = orc.get_fields()
+    assert np.max(abs(a - c)) < 1e-12 and np.max(abs(b - d)) < 1e-12
+    eng.close()
+
+
+def test_noise_free_update_matches(oracle_lib):
+    print("case: noise-free update", flush=True)
+    shape = (32, 8, 1, 16)
+    eng, orc, p = _pair(oracle_lib, shape, noise_on=0, factor=0.5, q3_mode=CLV_Q3_PSIBAR)
+    rng = np.random.default_rng(4)
+    d = eng.dims(0)
+    f = 1 + 0.3 * (rng.normal(size=d) + 1j * rng.normal(size=d))
+    g = 1 + 0.3 * (rng.normal(size=d) + 1j * rng.normal(size=d))
+    eng.set_fields(0, f, g)
+    orc.set_fields(f, g)
+    eng.apply_update(0, 1.3e-3)
+    orc.apply_update(1.3e-3)
+    a, b = eng.get_fields(0)
+    c, e = orc.get_fields()
+    assert np.max(abs(a - c)) < 1e-13 and np.max(abs(b - e)) < 1e-13
+    eng.close()
+
+
+@pytest.mark.parametrize("plain", [False, True])
+def test_trajectory_fp64_1000_steps(oracle_lib, plain):
+    """configs[0] shape (1-D 64 x 32), 1000 adaptive steps, per-site 1e-10."""
+    print("case: fp64 trajectory plain=%s" % plain, flush=True)
+    shape = (64, 1, 1, 32)
+    eng, orc, p = _pair(oracle_lib, shape, seed=1)
+    eng.init_fields(0, 1.0, 0.1)
+    orc.init_fields(1.0, 0.1)
+    if plain:
+        eng.propagate_plain(0, 1000, 10)
+    else:
+        eng.propagate(1000, 10)
+    orc.propagate(1000, 10)
+    a, b = eng.get_fields(0)
+    c, d = orc.get_fields()
+    err = max(np.max(abs(a - c)), np.max(abs(b - d)))
+    print("max per-site |delta| =", err)
+    assert err <= 1e-10
+    sg, so = eng.stats(0), orc.stats()
+    assert sg.steps == so.steps == 1000
+    assert sg.t_langevin == pytest.approx(so.t_langevin, rel=1e-12)
+    og, Fg = eng.observables(0)
+    oo, Fo = orc.observables()
+    assert og.count == oo.count == 100
+    assert og.n_tot == pytest.approx(oo.n_tot, rel=1e-11)
+    assert og.energy == pytest.approx(oo.energy, rel=1e-10)
+    for a_ in range(3):
+        assert np.max(abs(Fg[a_] - Fo[a_])) < 1e-10
+    eng.close()
+
+
+def test_trajectory_fp32_1000_steps(oracle_lib):
+    """FP32 fields (FP64 dt, sqrt(dt), reductions) within 1e-4 relative."""
+    print("case: fp32 trajectory", flush=True)
+    shape = (64, 1, 1, 32)
+    eng, orc, p = _pair(oracle_lib, shape, precision=CLV_FP32, seed=1)
+    eng.init_fields(0, 1.0, 0.1)
+    orc.init_fields(1.0, 0.1)
+    eng.propagate(1000, 10)
+    orc.propagate(1000, 10)
+    a, b = eng.get_fields(0)
+    c, d = orc.get_fields()
+    scale = np.sqrt(np.mean(abs(c) ** 2))
+    rel = max(np.max(abs(a - c)), np.max(abs(b - d))) / scale
+    print("max per-site |delta| / rms =", rel)
+    assert rel <= 1e-4
+    og, _ = eng.observables(0)
+    oo, _ = orc.observables()
+    assert og.n_tot == pytest.approx(oo.n_tot, rel=1e-4)
+    eng.close()
+
+
+def test_observables_single_evaluation(oracle_lib):
+    print("case: observables", flush=True)
+    for shape in [(16, 16, 1, 8), (8, 4, 4, 4), (24, 6, 1, 4)]:
+        eng, orc, p = _pair(oracle_lib, shape, c1=0.07, P=0.1)
+        rng = np.random.default_rng(9)
+        d = eng.dims(0)
+        f = 1 + 0.3 * (rng.normal(size=d) + 1j * rng.normal(size=d))
+        g = 1 + 0.3 * (rng.normal(size=d) + 1j * rng.normal(size=d))
+        eng.set_fields(0, f, g)
+        orc.set_fields(f, g)
+        eng.evaluate(0)
+        orc.evaluate()
+        og, Fg = eng.observables(0)
+        oo, Fo = orc.observables()
+        assert og.count == 1
+        assert np.allclose(list(og.n_comp), list(oo.n_comp), rtol=1e-12)
+        assert og.energy == pytest.approx(oo.energy, rel=1e-12)
+        for a_ in range(3):
+            assert np.max(abs(Fg[a_] - Fo[a_])) < 1e-12
+        eng.close()
+
+
+def test_batch_same_alone_and_in_the_mix():
+    """Chains of different shapes and parameters in one context give bit-identical
+    results to the same chains run alone."""
+    print("case: batch mix", flush=True)
+    shapes = [(16, 16, 1, 8), (64, 1, 1, 32), (16, 16, 1, 8), (8, 4, 4, 4)]
+    mus = [0.5, 1.0, 1.5, 0.8]
+    for prec in (CLV_FP64, CLV_FP32):
+        mix = _engine(shapes, prec)
+        for i, (s, mu) in enumerate(zip(shapes, mus)):
+            mix.set_params(i, _params(seed=100 + i, mu=mu))
+            mix.init_fields(i, 1.0, 0.1)
+        mix.propagate(50, 5)
+        for i, (s, mu) in enumerate(zip(shapes, mus)):
+            one = _engine([s], prec)
+            one.set_params(0, _params(seed=100 + i, mu=mu))
+            one.init_fields(0, 1.0, 0.1)
+            one.propagate(50, 5)
+            a, b = mix.get_fields(i)
+            c, d = one.get_fields(0)
+            assert np.array_equal(a, c) and np.array_equal(b, d), (prec, i)
+            oa, _ = mix.observables(i)
+            ob, _ = one.observables(0)
+            assert oa.n_tot == ob.n_tot and oa.energy == ob.energy
+            one.close()
+        mix.close()
+
+
+def test_device_controller_equals_host_controller():
+    """The device-resident dt and the plain method's host dt agree bit for bit."""
+    print("case: controller", flush=True)
+    e1 = _engine([(32, 8, 1, 16)])
+    e2 = _engine([(32, 8, 1, 16)])
+    for e in (e1, e2):
+        e.set_params(0, _params(seed=5))
+        e.init_fields(0, 1.0, 0.1)
+    e1.propagate(20)
+    e2.propagate_plain(0, 20)
+    s1, s2 = e1.stats(0), e2.stats(0)
+    assert s1.dt == s2.dt and s1.t_langevin == s2.t_langevin
+    a, _ = e1.get_fields(0)
+    b, _ = e2.get_fields(0)
+    assert np.array_equal(a, b)
+    e1.close()
+    e2.close()
