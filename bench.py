@@ -129,6 +129,8 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--quiet", action="store_true")
+    ap.add_argument("--mode", choices=["all", "batched"], default="all",
+                    help="batched: only the batched leg (for profiling runs)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -146,6 +148,9 @@ def main() -> int:
         return 1
 
     b = run_batched(args, log)
+    if args.mode == "batched":
+        print(json.dumps({"batched_su_per_s": b["su"], "kernel_ms": b["kt"]}), flush=True)
+        return 0
     p = run_plain(args, log)
     # roofline of the dominant kernel: algorithmic bytes per launch / average launch time
     esz = 8  # complex64

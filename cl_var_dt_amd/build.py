@@ -18,7 +18,7 @@ HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
 
 # -packed-fp32-ops off: no v_pk_{add,mul,fma}_f32 in the generated code.
 HIP_FLAGS = [
-    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=on",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-function", "-Wno-unused-result",
 ]
@@ -44,7 +44,7 @@ def build_driver() -> str | None:
         return None
     out = os.path.join(PKG, "driver", "clv_main")
     _run(["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"),
-          "-o", out, src, "-L", PKG, "-lclv", f"-Wl,-rpath,{PKG}"])
+          "-o", out, src, "-L", PKG, "-lclv", "-Wl,-rpath,$ORIGIN/.."])
     return out
 
 

@@ -1,7 +1,7 @@
 // clv_device.h -- device building blocks of the gfx950 complex-Langevin engine:
 // complex arithmetic, Philox4x32-10 + Box-Muller noise, the adaptive-dt
-// controller, wave/workgroup reductions, and the LDS Stockham FFT that
-// produces the kinetic term of a row of X sites.
+// controller, wave/workgroup reductions, and the two-level LDS/register FFT
+// that produces the kinetic term of a row of X sites.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -96,26 +96,25 @@ __device__ __forceinline__ void normals4(float out[4], uint64_t seed, uint32_t s
                                          uint64_t step, uint32_t site, uint32_t pair) {
   const uint4 r = philox10(make_uint4(site, pair | (stream << 8), uint32_t(step), uint32_t(step >> 32)),
                            uint32_t(seed), uint32_t(seed >> 32));
+  // FP32 uniforms from the same 32-bit words as the FP64 path; the hardware
+  // transcendentals: v_log_f32 is log2, v_sin/v_cos take revolutions.
   const float s = 2.3283064365386963e-10f;
-  const float u0 = float(r.x) * s + 1.16415321826934814e-10f;
-  const float u1 = float(r.y) * s + 1.16415321826934814e-10f;
-  const float u2 = float(r.z) * s + 1.16415321826934814e-10f;
-  const float u3 = float(r.w) * s + 1.16415321826934814e-10f;
-  const float r0 = sqrtf(-2.0f * logf(u0));
-  const float r1 = sqrtf(-2.0f * logf(u2));
-  float s0, c0, s1, c1;
-  sincosf(6.28318530717958648f * u1, &s0, &c0);
-  sincosf(6.28318530717958648f * u3, &s1, &c1);
-  out[0] = r0 * c0;
-  out[1] = r0 * s0;
-  out[2] = r1 * c1;
-  out[3] = r1 * s1;
+  const float h = 1.16415321826934814e-10f;
+  const float u0 = fmaf(float(r.x), s, h), u1 = fmaf(float(r.y), s, h);
+  const float u2 = fmaf(float(r.z), s, h), u3 = fmaf(float(r.w), s, h);
+  const float r0 = __builtin_amdgcn_sqrtf(-1.38629436111989061f * __builtin_amdgcn_logf(u0));
+  const float r1 = __builtin_amdgcn_sqrtf(-1.38629436111989061f * __builtin_amdgcn_logf(u2));
+  out[0] = r0 * __builtin_amdgcn_cosf(u1);
+  out[1] = r0 * __builtin_amdgcn_sinf(u1);
+  out[2] = r1 * __builtin_amdgcn_cosf(u3);
+  out[3] = r1 * __builtin_amdgcn_sinf(u3);
 }
 
 // ------------------------------------------------------------- controller --
-// Propagator::compute_new_dt (Propagator_without_cutoff.cu:382-404), with
-// explicit IEEE operations so the device and host results are identical.
+// Propagator::compute_new_dt (Propagator_without_cutoff.cu:382-404).  No FMA
+// contraction, so the device and host (clv_compute_new_dt) results are identical.
 __device__ __forceinline__ double new_dt(double mean, double mx, const ChainParams &p, int *clamp) {
+#pragma clang fp contract(off)
   double dt = (mx <= 0.0) ? p.DT : __dmul_rn(__ddiv_rn(mean, mx), p.DT);
   int c = 0;
   if (dt < p.dt_min) { dt = p.dt_min; c = -1; }
@@ -136,22 +135,27 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
-// --------------------------------------------------------------- FFT plan --
-// Radix schedule of the row transform of length X (Stockham stages).
-template <int X> struct Plan;
-template <> struct Plan<4>   { static constexpr int NS = 1, R0 = 4, R1 = 1, R2 = 1; };
-template <> struct Plan<8>   { static constexpr int NS = 1, R0 = 8, R1 = 1, R2 = 1; };
-template <> struct Plan<16>  { static constexpr int NS = 2, R0 = 4, R1 = 4, R2 = 1; };
-template <> struct Plan<24>  { static constexpr int NS = 2, R0 = 8, R1 = 3, R2 = 1; };
-template <> struct Plan<32>  { static constexpr int NS = 2, R0 = 8, R1 = 4, R2 = 1; };
-template <> struct Plan<48>  { static constexpr int NS = 3, R0 = 4, R1 = 4, R2 = 3; };
-template <> struct Plan<64>  { static constexpr int NS = 2, R0 = 8, R1 = 8, R2 = 1; };
-template <> struct Plan<96>  { static constexpr int NS = 3, R0 = 8, R1 = 4, R2 = 3; };
-template <> struct Plan<128> { static constexpr int NS = 3, R0 = 8, R1 = 4, R2 = 4; };
-template <> struct Plan<192> { static constexpr int NS = 3, R0 = 8, R1 = 8, R2 = 3; };
-template <> struct Plan<256> { static constexpr int NS = 3, R0 = 8, R1 = 8, R2 = 4; };
+// ------------------------------------------------------------ row FFT ----
+// Exact twiddle constants of the small DFTs: cos / sin (2 pi m / R).
+__host__ __device__ constexpr double kcos(int m, int R) {
+  constexpr double c16[16] = {1.0, 0.92387953251128674, 0.70710678118654752, 0.38268343236508977,
+                              0.0, -0.38268343236508977, -0.70710678118654752, -0.92387953251128674,
+                              -1.0, -0.92387953251128674, -0.70710678118654752, -0.38268343236508977,
+                              0.0, 0.38268343236508977, 0.70710678118654752, 0.92387953251128674};
+  constexpr double c12[12] = {1.0, 0.86602540378443865, 0.5, 0.0, -0.5, -0.86602540378443865,
+                              -1.0, -0.86602540378443865, -0.5, 0.0, 0.5, 0.86602540378443865};
+  constexpr double c6[6] = {1.0, 0.5, -0.5, -1.0, -0.5, 0.5};
+  m = ((m % R) + R) % R;
+  return R == 16 ? c16[m] : R == 12 ? c12[m] : R == 8 ? c16[2 * m] : R == 6 ? c6[m] : R == 4 ? c16[4 * m] : 1.0;
+}
+__host__ __device__ constexpr double ksin(int m, int R) {
+  // sin(2 pi m / R) = cos(2 pi (m - R/4) / R) for R divisible by 4
+  constexpr double s6[6] = {0.0, 0.86602540378443865, 0.86602540378443865, 0.0,
+                            -0.86602540378443865, -0.86602540378443865};
+  return R == 6 ? s6[((m % 6) + 6) % 6] : kcos(m - R / 4, R);
+}
 
-// Small DFTs, y_q = sum_r x_r exp(S 2 pi i r q / R).
+// Small in-register DFTs, y_q = sum_r x_r exp(S 2 pi i r q / R).
 template <int S, typename T>
 __device__ __forceinline__ void dft2(C2<T> *x) {
   const C2<T> a = x[0], b = x[1];
@@ -160,7 +164,7 @@ __device__ __forceinline__ void dft2(C2<T> *x) {
 }
 template <int S, typename T>
 __device__ __forceinline__ void dft3(C2<T> *x) {
-  const T h = T(0.86602540378443864676);  // sqrt(3)/2
+  const T h = T(0.86602540378443865);  // sqrt(3)/2
   const C2<T> t = cadd(x[1], x[2]);
   const C2<T> d = csub(x[1], x[2]);
   const C2<T> u = cmul_si<S>(cscale(h, d));
@@ -178,100 +182,133 @@ __device__ __forceinline__ void dft4(C2<T> *x) {
   x[1] = cadd(t1, t3);
   x[3] = csub(t1, t3);
 }
-template <int S, typename T>
-__device__ __forceinline__ void dft8(C2<T> *x) {
-  C2<T> e[4] = {x[0], x[2], x[4], x[6]};
-  C2<T> o[4] = {x[1], x[3], x[5], x[7]};
-  dft4<S>(e);
-  dft4<S>(o);
-  const T r = T(0.70710678118654752440);
-  // w8^k o_k, w8 = (1 + S i)/sqrt(2)
-  const C2<T> o1 = {r * (o[1].x - T(S) * o[1].y), r * (o[1].y + T(S) * o[1].x)};
-  const C2<T> o2 = cmul_si<S>(o[2]);
-  const C2<T> o3 = {r * (-o[3].x - T(S) * o[3].y), r * (-o[3].y + T(S) * o[3].x)};
-  x[0] = cadd(e[0], o[0]);
-  x[4] = csub(e[0], o[0]);
-  x[1] = cadd(e[1], o1);
-  x[5] = csub(e[1], o1);
-  x[2] = cadd(e[2], o2);
-  x[6] = csub(e[2], o2);
-  x[3] = cadd(e[3], o3);
-  x[7] = csub(e[3], o3);
+template <int R, int S, typename T>
+__device__ __forceinline__ void dft(C2<T> *x);
+
+// R = A*B: n = n2 + B n1, k = k1 + A k2 (decimation as in Cooley-Tukey, 1965).
+template <int A, int B, int S, typename T>
+__device__ __forceinline__ void dft_split(C2<T> *x) {
+  C2<T> t[A * B];
+#pragma unroll
+  for (int n2 = 0; n2 < B; ++n2) {
+    C2<T> v[A];
+#pragma unroll
+    for (int n1 = 0; n1 < A; ++n1) v[n1] = x[n2 + B * n1];
+    dft<A, S>(v);
+#pragma unroll
+    for (int k1 = 0; k1 < A; ++k1) {
+      const int m = n2 * k1;
+      if (m % (A * B) == 0) {
+        t[n2 * A + k1] = v[k1];
+      } else {
+        const C2<T> w = {T(kcos(m, A * B)), T(S * ksin(m, A * B))};
+        t[n2 * A + k1] = cmul(v[k1], w);
+      }
+    }
+  }
+#pragma unroll
+  for (int k1 = 0; k1 < A; ++k1) {
+    C2<T> v[B];
+#pragma unroll
+    for (int n2 = 0; n2 < B; ++n2) v[n2] = t[n2 * A + k1];
+    dft<B, S>(v);
+#pragma unroll
+    for (int k2 = 0; k2 < B; ++k2) x[k1 + A * k2] = v[k2];
+  }
 }
+
 template <int R, int S, typename T>
 __device__ __forceinline__ void dft(C2<T> *x) {
-  if constexpr (R == 2) dft2<S>(x);
-  else if constexpr (R == 3) dft3<S>(x);
-  else if constexpr (R == 4) dft4<S>(x);
-  else if constexpr (R == 8) dft8<S>(x);
-}
-
-// One Stockham stage over NL lines of length X held in LDS (line l at
-// src + l*X): for i < X/R, k = i mod P, x_r = src[i + r X/R] w^{S r k X/(P R)},
-// DFT_R, dst[(i-k) R + k + r P] = y_r.  tw[m] = exp(-2 pi i m / X).
-template <typename T, int X, int R, int P, int S>
-__device__ __forceinline__ void fft_stage(const C2<T> *__restrict__ src, C2<T> *__restrict__ dst,
-                                          const C2<T> *__restrict__ tw, int nlines, int tid,
-                                          int nthr) {
-  constexpr int NB = X / R;
-  const int total = nlines * NB;
-  for (int b = tid; b < total; b += nthr) {
-    const int line = b / NB;
-    const int i = b - line * NB;
-    const int k = i % P;
-    const C2<T> *s = src + line * X;
-    C2<T> v[R];
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-      C2<T> a = s[i + r * NB];
-      if (P > 1 && r > 0) {
-        const C2<T> w = tw[(r * k * (X / (P * R))) % X];
-        a = cmul(a, S < 0 ? w : cconj(w));
-      }
-      v[r] = a;
-    }
-    dft<R, S>(v);
-    C2<T> *d = dst + line * X + (i - k) * R + k;
-#pragma unroll
-    for (int r = 0; r < R; ++r) d[r * P] = v[r];
+  if constexpr (R == 1) {
+  } else if constexpr (R == 2) {
+    dft2<S>(x);
+  } else if constexpr (R == 3) {
+    dft3<S>(x);
+  } else if constexpr (R == 4) {
+    dft4<S>(x);
+  } else if constexpr (R == 6) {
+    dft_split<3, 2, S>(x);
+  } else if constexpr (R == 8) {
+    dft_split<4, 2, S>(x);
+  } else if constexpr (R == 12) {
+    dft_split<4, 3, S>(x);
+  } else if constexpr (R == 16) {
+    dft_split<4, 4, S>(x);
   }
 }
 
-template <typename T, int X, int S>
-__device__ __forceinline__ C2<T> *fft_lines(C2<T> *a, C2<T> *b, const C2<T> *tw, int nlines,
-                                            int tid, int nthr) {
-  using PL = Plan<X>;
-  fft_stage<T, X, PL::R0, 1, S>(a, b, tw, nlines, tid, nthr);
-  __syncthreads();
-  if constexpr (PL::NS == 1) {
-    return b;
-  } else {
-    fft_stage<T, X, PL::R1, PL::R0, S>(b, a, tw, nlines, tid, nthr);
-    __syncthreads();
-    if constexpr (PL::NS == 2) {
-      return a;
-    } else {
-      fft_stage<T, X, PL::R2, PL::R0 * PL::R1, S>(a, b, tw, nlines, tid, nthr);
-      __syncthreads();
-      return b;
-    }
-  }
-}
+// Row length X = N1 * N2 of the two-level transform.
+template <int X> struct Split;
+template <> struct Split<4>   { static constexpr int N1 = 2, N2 = 2; };
+template <> struct Split<8>   { static constexpr int N1 = 4, N2 = 2; };
+template <> struct Split<16>  { static constexpr int N1 = 4, N2 = 4; };
+template <> struct Split<24>  { static constexpr int N1 = 6, N2 = 4; };
+template <> struct Split<32>  { static constexpr int N1 = 8, N2 = 4; };
+template <> struct Split<48>  { static constexpr int N1 = 8, N2 = 6; };
+template <> struct Split<64>  { static constexpr int N1 = 8, N2 = 8; };
+template <> struct Split<96>  { static constexpr int N1 = 12, N2 = 8; };
+template <> struct Split<128> { static constexpr int N1 = 16, N2 = 8; };
+template <> struct Split<192> { static constexpr int N1 = 16, N2 = 12; };
+template <> struct Split<256> { static constexpr int N1 = 16, N2 = 16; };
 
-// Kinetic term of NL lines held in LDS buffer a (b is scratch):
-// K = F^{+}[m(k) F^{-}[line]] with m(k) = -p_x(k)^2 / (a_x X).  Returns the
-// buffer holding K (natural order).  Caller syncs before the call.
+// Kinetic term of rows of X sites held in LDS, in place, by a two-level
+// (four-step) FFT: x[n2 + N2 n1] -> DFT_N1 over n1, twiddle W_X^{n2 k1} ->
+// DFT_N2 over n2 gives X[k1 + N1 k2]; multiply by m(k) = -p_x(k)^2/(a_x X);
+// the inverse runs the same two levels backwards and leaves x in natural
+// order.  Each level reads and writes the same LDS cells, so one buffer
+// suffices; a line is padded by one cell per N2 so that both access patterns
+// are free of bank conflicts.
 template <typename T, int X>
-__device__ __forceinline__ C2<T> *kinetic_lines(C2<T> *a, C2<T> *b, const C2<T> *tw,
-                                                const T *mult, int nlines, int tid, int nthr) {
-  C2<T> *f = fft_lines<T, X, -1>(a, b, tw, nlines, tid, nthr);
-  for (int e = tid; e < nlines * X; e += nthr) {
-    const T m = mult[e % X];
-    f[e] = cscale(m, f[e]);
+struct RowFFT {
+  static constexpr int N1 = Split<X>::N1, N2 = Split<X>::N2;
+  static constexpr int PAD = X + X / N2;  // LDS cells per line
+  __device__ static __forceinline__ int pos(int p) { return p + p / N2; }
+
+  // lines l < nl at buf + l*PAD; tw[m] = exp(-2 pi i m / X); mult[k] = m(k).
+  // Caller syncs before (data written) and reads after the trailing sync.
+  __device__ static void kinetic(C2<T> *buf, const C2<T> *tw, const T *mult, int nl, int tid,
+                                 int nthr) {
+    for (int task = tid; task < nl * N2; task += nthr) {
+      const int l = task / N2, n2 = task - (task / N2) * N2;
+      C2<T> *L = buf + l * PAD;
+      C2<T> v[N1];
+#pragma unroll
+      for (int n1 = 0; n1 < N1; ++n1) v[n1] = L[pos(n2 + N2 * n1)];
+      dft<N1, -1>(v);
+#pragma unroll
+      for (int k1 = 1; k1 < N1; ++k1) v[k1] = cmul(v[k1], tw[(n2 * k1) % X]);
+#pragma unroll
+      for (int k1 = 0; k1 < N1; ++k1) L[pos(n2 + N2 * k1)] = v[k1];
+    }
+    __syncthreads();
+    for (int task = tid; task < nl * N1; task += nthr) {
+      const int l = task / N1, k1 = task - (task / N1) * N1;
+      C2<T> *L = buf + l * PAD;
+      C2<T> v[N2];
+#pragma unroll
+      for (int n2 = 0; n2 < N2; ++n2) v[n2] = L[pos(n2 + N2 * k1)];
+      dft<N2, -1>(v);
+#pragma unroll
+      for (int k2 = 0; k2 < N2; ++k2) v[k2] = cscale(mult[k1 + N1 * k2], v[k2]);
+      dft<N2, +1>(v);
+#pragma unroll
+      for (int m2 = 1; m2 < N2; ++m2) v[m2] = cmul(v[m2], cconj(tw[(m2 * k1) % X]));
+#pragma unroll
+      for (int m2 = 0; m2 < N2; ++m2) L[pos(m2 + N2 * k1)] = v[m2];
+    }
+    __syncthreads();
+    for (int task = tid; task < nl * N2; task += nthr) {
+      const int l = task / N2, m2 = task - (task / N2) * N2;
+      C2<T> *L = buf + l * PAD;
+      C2<T> v[N1];
+#pragma unroll
+      for (int k1 = 0; k1 < N1; ++k1) v[k1] = L[pos(m2 + N2 * k1)];
+      dft<N1, +1>(v);
+#pragma unroll
+      for (int m1 = 0; m1 < N1; ++m1) L[pos(m2 + N2 * m1)] = v[m1];
+    }
+    __syncthreads();
   }
-  __syncthreads();
-  C2<T> *g = (f == a) ? b : a;
-  return fft_lines<T, X, +1>(f, g, tw, nlines, tid, nthr);
-}
+};
 
 }  // namespace clv

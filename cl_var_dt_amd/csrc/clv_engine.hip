@@ -54,30 +54,21 @@ __global__ __launch_bounds__(256) void k_drift_stats(BucketArgs<T> A) {
     load4(psi, (int64_t(tm1) * A.R + r) * X + x, pm);
     load4(psib, (int64_t(t) * A.R + r) * X + x, b0);
     p00 = load1(psi, (int64_t(t) * A.R + r) * X + x, 0);
-    sm.a[j * X + x] = pm[0];
+    sm.at(j, x) = pm[0];
   }
   __syncthreads();
-  const C2<T> *K = kinetic_lines<T, X>(sm.a, sm.b, sm.tw, sm.mult, A.rows, tid, nthr);
+  RowFFT<T, X>::kinetic(sm.buf, sm.tw, sm.mult, A.rows, tid, nthr);
   double v = 0.0;
   if (has_site) {
     const Coef<T> k = coef<T>(P);
-    const C2<T> d = drift0_site(k, pm, p00, b0, K[j * X + x]);
+    const C2<T> d = drift0_site(k, pm, p00, b0, sm.at(j, x));
     const double dx = double(d.x), dy = double(d.y);
     v = sqrt(dx * dx + dy * dy);
   }
-  double m = wave_max(v), s = wave_sum(v);
-  const int w = tid >> 6, nw = (nthr + 63) >> 6;
-  if ((tid & 63) == 0) {
-    sm.red[w] = m;
-    sm.red[32 + w] = s;
-  }
-  __syncthreads();
+  double m, s;
+  block_max_sum(v, sm.red, tid, nthr, &m, &s);
   if (tid == 0) {
-    for (int q = 1; q < nw; ++q) {
-      m = fmax(m, sm.red[q]);
-      s += sm.red[32 + q];
-    }
-    double *o = A.partials + ((int64_t(chain) * A.Nt + t) * A.ntiles + tile) * 2;
+    double *o = A.part_wr + ((int64_t(chain) * A.Nt + t) * A.ntiles + tile) * 2;
     o[0] = m;
     o[1] = s;
   }
@@ -97,14 +88,8 @@ __global__ __launch_bounds__(64) void k_controller(BucketArgs<T> A, int mode, do
   const int lane = threadIdx.x;
   if (mode != 2) {
     const int n = A.Nt * A.ntiles;
-    const double *pp = A.partials + int64_t(chain) * n * 2;
-    double m = 0.0, s = 0.0;
-    for (int i = lane; i < n; i += 64) {
-      m = fmax(m, pp[2 * i]);
-      s += pp[2 * i + 1];
-    }
-    m = wave_max(m);
-    s = wave_sum(s);
+    double m, s;
+    reduce_partials(A.part_rd + int64_t(chain) * n * 2, n, lane, &m, &s);
     if (lane == 0) {
       const double mean = s / double(4 * A.S);
       st.max_drift = m;
@@ -148,7 +133,7 @@ __global__ __launch_bounds__(256) void k_update(BucketArgs<T> A) {
   const int tid = threadIdx.x, nthr = blockDim.x;
   const int j = tid / X, x = tid - (tid / X) * X;
   const bool has_site = j < A.rows;
-  Smem<T, X> sm(smem_raw, 4 * A.rows);
+  Smem<T, X> sm(smem_raw, 8 * A.rows);
   load_tables<T, X>(sm, A.tw, P.a_x, tid, nthr);
   const ChainState &st = A.st[chain];
   const int par = st.in_parity;
@@ -158,36 +143,34 @@ __global__ __launch_bounds__(256) void k_update(BucketArgs<T> A) {
   const C2<T> *psib = (par ? A.buf1_psib : A.buf0_psib) + chain * A.chain_stride;
   C2<T> *opsi = (par ? A.buf0_psi : A.buf1_psi) + chain * A.chain_stride;
   C2<T> *opsib = (par ? A.buf0_psib : A.buf1_psib) + chain * A.chain_stride;
-  const int r = tile * A.rows + j;
+  const int rows = A.rows;
+  const int r = tile * rows + j;
   const int tm1 = (t == 0) ? A.Nt - 1 : t - 1;
   const int tp1 = (t == A.Nt - 1) ? 0 : t + 1;
   C2<T> pm[4], p0[4], pp[4], b0[4], bp[4], km[4], kp[4];
   const int64_t s0 = (int64_t(t) * A.R + r) * X + x;
+  const int64_t sp1 = (int64_t(tp1) * A.R + r) * X + x;
+  // K(t-1) of psi and Kbar(t+1) of psibar: one batch of 8 row sets
   if (has_site) {
     load4(psi, (int64_t(tm1) * A.R + r) * X + x, pm);
-    load4(psi, s0, p0);
-    load4(psi, (int64_t(tp1) * A.R + r) * X + x, pp);
-    load4(psib, s0, b0);
-    load4(psib, (int64_t(tp1) * A.R + r) * X + x, bp);
+    load4(psib, sp1, bp);
 #pragma unroll
-    for (int c = 0; c < 4; ++c) sm.a[(c * A.rows + j) * X + x] = pm[c];
+    for (int c = 0; c < 4; ++c) {
+      sm.at(c * rows + j, x) = pm[c];
+      sm.at((4 + c) * rows + j, x) = bp[c];
+    }
   }
   __syncthreads();
-  const C2<T> *K = kinetic_lines<T, X>(sm.a, sm.b, sm.tw, sm.mult, 4 * A.rows, tid, nthr);
-  if (has_site) {
-#pragma unroll
-    for (int c = 0; c < 4; ++c) km[c] = K[(c * A.rows + j) * X + x];
-  }
-  __syncthreads();
-  if (has_site) {
-#pragma unroll
-    for (int c = 0; c < 4; ++c) sm.a[(c * A.rows + j) * X + x] = bp[c];
-  }
-  __syncthreads();
-  K = kinetic_lines<T, X>(sm.a, sm.b, sm.tw, sm.mult, 4 * A.rows, tid, nthr);
+  RowFFT<T, X>::kinetic(sm.buf, sm.tw, sm.mult, 8 * rows, tid, nthr);
   if (!has_site) return;
 #pragma unroll
-  for (int c = 0; c < 4; ++c) kp[c] = K[(c * A.rows + j) * X + x];
+  for (int c = 0; c < 4; ++c) {
+    km[c] = sm.at(c * rows + j, x);
+    kp[c] = sm.at((4 + c) * rows + j, x);
+  }
+  load4(psi, s0, p0);
+  load4(psi, sp1, pp);
+  load4(psib, s0, b0);
   const Coef<T> k = coef<T>(P);
   C2<T> D[4], Db[4];
   drift_site(k, pm, p0, pp, b0, bp, km, kp, D, Db);
@@ -210,6 +193,183 @@ __global__ __launch_bounds__(256) void k_update(BucketArgs<T> A) {
   }
   store4(opsi, s0, o);
   store4(opsib, s0, ob);
+}
+
+// Fused single-pass step (schedule 2).  One workgroup owns `rows` rows of one
+// chain for all Nt slices and marches along tau, updating in place:
+//   1. dt of this step from the partials of the current fields (the same
+//      fixed-order reduction as k_controller; workgroup 0 commits the state),
+//   2. per tau: K(t) and Kbar(t+1) of the old fields and K_0 of the new
+//      psi_0(t-1) in one LDS batch of 9 row sets, drift + Philox noise +
+//      update, and |D_0| of the NEW fields at t-1 -> next step's partials.
+// The new tau = 0 slice is held in registers until the old one has been read
+// as the wrap-around neighbour of tau = Nt-1.  Reads and writes the state once
+// per step; results are bit-identical to the two-pass schedule.
+template <typename T, int X>
+__global__ __launch_bounds__(256) void k_fused(BucketArgs<T> A) {
+  extern __shared__ __align__(16) char smem_raw[];
+  const int chain = A.chain0 + blockIdx.y;
+  const int tile = blockIdx.x;
+  const int tid = threadIdx.x, nthr = blockDim.x;
+  const ChainParams &P = A.prm[chain];
+  const ChainState &st = A.st[chain];
+  if (!P.active) {
+    if (tile == 0 && tid == 0) A.st_next[chain] = st;
+    return;
+  }
+  const int rows = A.rows, Nt = A.Nt;
+  Smem<T, X> sm(smem_raw, 9 * rows);
+  using RF = RowFFT<T, X>;
+  load_tables<T, X>(sm, A.tw, P.a_x, tid, nthr);
+  if (tid < 64) {
+    const int n = Nt * A.ntiles;
+    double m, s;
+    reduce_partials(A.part_rd + int64_t(chain) * n * 2, n, tid, &m, &s);
+    if (tid == 0) {
+      const double mean = s / double(4 * A.S);
+      int clamp = 0;
+      const double dt = new_dt(mean, m, P, &clamp);
+      sm.red[0] = dt;
+      if (tile == 0) {
+        ChainState ns = st;
+        ns.dt = dt;
+        ns.max_drift = m;
+        ns.mean_drift = mean;
+        if (!isfinite(m) || !isfinite(s)) ns.nonfinite = 1;
+        ns.cur_step = st.steps;
+        ns.steps = st.steps + 1;
+        ns.t_langevin = st.t_langevin + dt;
+        if (clamp < 0) ns.clamp_low += 1;
+        if (clamp > 0) ns.clamp_high += 1;
+        ns.in_parity = st.parity;
+        A.st_next[chain] = ns;
+      }
+    }
+  }
+  __syncthreads();
+  const double dt = sm.red[0];
+  const uint64_t step = uint64_t(st.steps);
+  const int j = tid / X, x = tid - (tid / X) * X;
+  const bool has_site = j < rows;
+  const int r = tile * rows + j;
+  const int par = st.parity;
+  C2<T> *psi = (par ? A.buf1_psi : A.buf0_psi) + chain * A.chain_stride;
+  C2<T> *psib = (par ? A.buf1_psib : A.buf0_psib) + chain * A.chain_stride;
+  const Coef<T> k = coef<T>(P);
+  const T dtT = T(dt);
+  const T sq = T(sqrt(dt));
+  C2<T> pm[4], p0[4], pp[4], b0[4], bp[4], km[4], kp[4];
+  C2<T> nprev[4], n0k[4], nb0k[4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) nprev[c] = {T(0), T(0)};
+  if (has_site) {
+    load4(psi, (int64_t(Nt - 1) * A.R + r) * X + x, pm);
+    load4(psi, int64_t(r) * X + x, p0);
+    load4(psib, int64_t(r) * X + x, b0);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) sm.at(c * rows + j, x) = pm[c];
+  }
+  __syncthreads();
+  RF::kinetic(sm.buf, sm.tw, sm.mult, 4 * rows, tid, nthr);
+  if (has_site) {
+#pragma unroll
+    for (int c = 0; c < 4; ++c) km[c] = sm.at(c * rows + j, x);
+  }
+  for (int t = 0; t < Nt; ++t) {
+    const int tp1 = (t == Nt - 1) ? 0 : t + 1;
+    if (has_site) {
+      load4(psi, (int64_t(tp1) * A.R + r) * X + x, pp);
+      load4(psib, (int64_t(tp1) * A.R + r) * X + x, bp);
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        sm.at(c * rows + j, x) = p0[c];
+        sm.at((4 + c) * rows + j, x) = bp[c];
+      }
+      sm.at(8 * rows + j, x) = nprev[0];
+    }
+    __syncthreads();
+    RF::kinetic(sm.buf, sm.tw, sm.mult, 9 * rows, tid, nthr);
+    double v = 0.0;
+    if (has_site) {
+      C2<T> kn[4];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        kn[c] = sm.at(c * rows + j, x);
+        kp[c] = sm.at((4 + c) * rows + j, x);
+      }
+      const C2<T> k0n = sm.at(8 * rows + j, x);
+      C2<T> D[4], Db[4];
+      drift_site(k, pm, p0, pp, b0, bp, km, kp, D, Db);
+      T nz[8] = {};
+      if (P.noise_on) {
+        const uint32_t site = uint32_t((int64_t(t) * X + x) * A.R + r);
+        normals4(nz, P.seed, 0u, step, site, 0u);
+        normals4(nz + 4, P.seed, 0u, step, site, 1u);
+#pragma unroll
+        for (int q = 0; q < 8; ++q) nz[q] *= sq;
+      }
+      C2<T> o[4], ob[4];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const C2<T> eta = {nz[2 * c], nz[2 * c + 1]};
+        o[c] = cadd(p0[c], cadd(cscale(dtT, D[c]), eta));
+        ob[c] = cadd(b0[c], cadd(cscale(dtT, Db[c]), cconj(eta)));
+      }
+      if (t == 0) {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          n0k[c] = o[c];
+          nb0k[c] = ob[c];
+        }
+      } else {
+        const int64_t s0 = (int64_t(t) * A.R + r) * X + x;
+        store4(psi, s0, o);
+        store4(psib, s0, ob);
+        const C2<T> d0 = drift0_site(k, nprev, o[0], ob, k0n);
+        const double dx = double(d0.x), dy = double(d0.y);
+        v = sqrt(dx * dx + dy * dy);
+      }
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        pm[c] = p0[c];
+        p0[c] = pp[c];
+        b0[c] = bp[c];
+        km[c] = kn[c];
+        nprev[c] = o[c];
+      }
+    }
+    if (t > 0) {
+      double m, s;
+      block_max_sum(v, sm.red, tid, nthr, &m, &s);
+      if (tid == 0) {
+        double *o = A.part_wr + ((int64_t(chain) * Nt + t) * A.ntiles + tile) * 2;
+        o[0] = m;
+        o[1] = s;
+      }
+    }
+  }
+  // tau = 0 of the new fields, and its |D_0| (needs the new psi(Nt-1))
+  if (has_site) {
+    const int64_t s0 = int64_t(r) * X + x;
+    store4(psi, s0, n0k);
+    store4(psib, s0, nb0k);
+    sm.at(j, x) = nprev[0];
+  }
+  __syncthreads();
+  RF::kinetic(sm.buf, sm.tw, sm.mult, rows, tid, nthr);
+  double v = 0.0;
+  if (has_site) {
+    const C2<T> d0 = drift0_site(k, nprev, n0k[0], nb0k, sm.at(j, x));
+    const double dx = double(d0.x), dy = double(d0.y);
+    v = sqrt(dx * dx + dy * dy);
+  }
+  double m, sum;
+  block_max_sum(v, sm.red, tid, nthr, &m, &sum);
+  if (tid == 0) {
+    double *o = A.part_wr + (int64_t(chain) * Nt * A.ntiles + tile) * 2;
+    o[0] = m;
+    o[1] = sum;
+  }
 }
 
 // Observables of the current fields over one tau-chunk: n_c (real space,
@@ -245,15 +405,15 @@ __global__ __launch_bounds__(256) void k_obs(BucketArgs<T> A) {
       load4(psi, (int64_t(t) * A.R + r) * X + x, q);
       load4(psib, (int64_t(tp1) * A.R + r) * X + x, b);
 #pragma unroll
-      for (int c = 0; c < 4; ++c) sm.a[(c * A.rows + j) * X + x] = q[c];
+      for (int c = 0; c < 4; ++c) sm.at(c * A.rows + j, x) = q[c];
     }
     __syncthreads();
-    const C2<T> *K = kinetic_lines<T, X>(sm.a, sm.b, sm.tw, sm.mult, 4 * A.rows, tid, nthr);
+    RowFFT<T, X>::kinetic(sm.buf, sm.tw, sm.mult, 4 * A.rows, tid, nthr);
     if (has_site) {
       C2<double> bb[4], qq[4], kk[4];
 #pragma unroll
       for (int c = 0; c < 4; ++c) {
-        const C2<T> kc = K[(c * A.rows + j) * X + x];
+        const C2<T> kc = sm.at(c * A.rows + j, x);
         bb[c] = {double(b[c].x), double(b[c].y)};
         qq[c] = {double(q[c].x), double(q[c].y)};
         kk[c] = {double(kc.x), double(kc.y)};
@@ -401,7 +561,11 @@ struct Bucket {
          *d_acc = nullptr;
   int64_t *d_count = nullptr;
   std::vector<int64_t> host_steps;
-  int schedule = 1;
+  int schedule = 1;      // 1 two-pass, 2 fused
+  int epoch = 0;         // slot (epoch & 1) of d_st / d_part is current
+  bool primed = false;   // current partials describe the current fields
+  size_t part_slot = 0;  // doubles per partials slot
+  size_t fused_lds = 0;  // LDS bytes of the fused kernel (0: not available)
 };
 
 }  // namespace
@@ -462,16 +626,19 @@ ChainParams to_device_params(const clv_params &p) {
 }
 
 template <typename T>
-BucketArgs<T> make_args(const Bucket &b, int chain0, int eval_every) {
+BucketArgs<T> make_args(const Bucket &b, int chain0, int eval_every, bool fused = false) {
   BucketArgs<T> A;
   A.buf0_psi = static_cast<C2<T> *>(b.psi[0]);
   A.buf0_psib = static_cast<C2<T> *>(b.psib[0]);
   A.buf1_psi = static_cast<C2<T> *>(b.psi[1]);
   A.buf1_psib = static_cast<C2<T> *>(b.psib[1]);
   A.prm = b.d_prm;
-  A.st = b.d_st;
+  const int slot = b.epoch & 1;
+  A.st = b.d_st + slot * b.nch;
+  A.st_next = fused ? b.d_st + (slot ^ 1) * b.nch : A.st;
   A.tw = static_cast<const C2<T> *>(b.d_tw);
-  A.partials = b.d_part;
+  A.part_rd = b.d_part + slot * b.part_slot;
+  A.part_wr = fused ? b.d_part + (slot ^ 1) * b.part_slot : A.part_rd;
   A.obs_stage = b.d_stage;
   A.obs_spart = b.d_spart;
   A.obs_F = b.d_F;
@@ -501,12 +668,21 @@ void launch_row_kernels(const Bucket &b, hipStream_t s, int which, int chain0, i
     const size_t sh = Smem<T, X>::bytes(b.rows);
     hipLaunchKernelGGL((k_drift_stats<T, X>), dim3(b.ntiles, b.Nt, nch), block, sh, s, A);
   } else if (which == 1) {
-    const size_t sh = Smem<T, X>::bytes(4 * b.rows);
+    const size_t sh = Smem<T, X>::bytes(8 * b.rows);
     hipLaunchKernelGGL((k_update<T, X>), dim3(b.ntiles, b.Nt, nch), block, sh, s, A);
-  } else {
+  } else if (which == 2) {
     const size_t sh = Smem<T, X>::bytes(4 * b.rows);
     hipLaunchKernelGGL((k_obs<T, X>), dim3(b.ntiles, b.nchunks, nch), block, sh, s, A);
+  } else {
+    BucketArgs<T> F = make_args<T>(b, chain0, eval_every, true);
+    const size_t sh = Smem<T, X>::bytes(9 * b.rows);
+    hipLaunchKernelGGL((k_fused<T, X>), dim3(b.ntiles, nch), block, sh, s, F);
   }
+}
+
+template <typename T, int X>
+size_t fused_lds_bytes(int rows) {
+  return Smem<T, X>::bytes(9 * rows);
 }
 
 template <typename T>
@@ -589,7 +765,8 @@ void drain_events(clv_ctx *c) {
 
 // precision-dispatching wrappers
 void row_kernel(clv_ctx *c, const Bucket &b, int which, int chain0, int nch, int ev) {
-  const int kind = which == 0 ? 0 : (which == 1 ? 2 : 3);
+  static const int kinds[4] = {0, 2, 3, 5};
+  const int kind = kinds[which];
   timed(c, kind, [&] {
     if (c->precision == CLV_FP32) dispatch_row<float>(b, c->stream, which, chain0, nch, ev);
     else dispatch_row<double>(b, c->stream, which, chain0, nch, ev);
@@ -606,6 +783,18 @@ void obs_finish(clv_ctx *c, const Bucket &b, int chain0, int nch, int ev) {
     if (c->precision == CLV_FP32) launch_obs_finish<float>(b, c->stream, chain0, nch, ev);
     else launch_obs_finish<double>(b, c->stream, chain0, nch, ev);
   });
+}
+
+// N2 of the two-level row transform (Split<X> in clv_device.h)
+int split_n2(int X) {
+  switch (X) {
+    case 4: case 8: return 2;
+    case 16: case 24: case 32: return 4;
+    case 48: return 6;
+    case 64: case 96: case 128: return 8;
+    case 192: return 12;
+    default: return 16;
+  }
 }
 
 // rows per workgroup: the largest divisor of R with X*rows <= 256
@@ -638,10 +827,10 @@ int setup_bucket(clv_ctx *ctx, Bucket &b) {
     if ((rc = dalloc(ctx, &b.psib[k], field))) return rc;
   }
   if ((rc = dalloc(ctx, reinterpret_cast<void **>(&b.d_prm), sizeof(ChainParams) * b.nch))) return rc;
-  if ((rc = dalloc(ctx, reinterpret_cast<void **>(&b.d_st), sizeof(ChainState) * b.nch))) return rc;
+  if ((rc = dalloc(ctx, reinterpret_cast<void **>(&b.d_st), 2 * sizeof(ChainState) * b.nch))) return rc;
   if ((rc = dalloc(ctx, &b.d_tw, b.esz * b.X))) return rc;
-  if ((rc = dalloc(ctx, reinterpret_cast<void **>(&b.d_part),
-                   sizeof(double) * 2 * b.nch * b.Nt * b.ntiles)))
+  b.part_slot = size_t(2) * b.nch * b.Nt * b.ntiles;
+  if ((rc = dalloc(ctx, reinterpret_cast<void **>(&b.d_part), sizeof(double) * 2 * b.part_slot)))
     return rc;
   if ((rc = dalloc(ctx, reinterpret_cast<void **>(&b.d_stage),
                    sizeof(double) * 3 * b.nch * b.nchunks * b.Vs)))
@@ -665,7 +854,7 @@ int setup_bucket(clv_ctx *ctx, Bucket &b) {
   } else {
     HIPCHK(ctx, hipMemcpy(b.d_tw, tw.data(), sizeof(double) * tw.size(), hipMemcpyHostToDevice));
   }
-  HIPCHK(ctx, hipMemset(b.d_st, 0, sizeof(ChainState) * b.nch));
+  HIPCHK(ctx, hipMemset(b.d_st, 0, 2 * sizeof(ChainState) * b.nch));
   HIPCHK(ctx, hipMemset(b.d_F, 0, sizeof(double) * 3 * b.nch * b.Vs));
   HIPCHK(ctx, hipMemset(b.d_acc, 0, sizeof(double) * 5 * b.nch));
   HIPCHK(ctx, hipMemset(b.d_count, 0, sizeof(int64_t) * b.nch));
@@ -674,6 +863,10 @@ int setup_bucket(clv_ctx *ctx, Bucket &b) {
     HIPCHK(ctx, hipMemset(b.psib[k], 0, field));
   }
   b.host_steps.assign(b.nch, 0);
+  const size_t lds = size_t(b.esz) * b.X + size_t(b.esz / 2) * (b.X + (b.X & 1)) +
+                     size_t(b.esz) * 9 * b.rows * (b.X + b.X / split_n2(b.X)) + sizeof(double) * 64;
+  b.fused_lds = (lds <= 65536) ? lds : 0;
+  b.schedule = 1;
   return CLV_OK;
 }
 
@@ -685,8 +878,8 @@ int check_chain(clv_ctx *ctx, int chain) {
 }
 
 int read_state(clv_ctx *ctx, const Bucket &b, int local, ChainState *out) {
-  HIPCHK(ctx, hipMemcpyAsync(out, b.d_st + local, sizeof(ChainState), hipMemcpyDeviceToHost,
-                             ctx->stream));
+  HIPCHK(ctx, hipMemcpyAsync(out, b.d_st + (b.epoch & 1) * b.nch + local, sizeof(ChainState),
+                             hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   return CLV_OK;
 }
@@ -883,6 +1076,7 @@ int clv_set_fields(clv_ctx *ctx, int32_t chain, const double *psi, const double 
     void *dst = static_cast<char *>(f ? b.psib[par] : b.psi[par]) + size_t(local) * n * b.esz;
     HIPCHK(ctx, hipMemcpy(dst, host.data(), n * b.esz, hipMemcpyHostToDevice));
   }
+  b.primed = false;
   return CLV_OK;
 }
 
@@ -932,15 +1126,24 @@ int clv_init_fields(clv_ctx *ctx, int32_t chain, double n0, double var) {
   Bucket &b = ctx->buckets[bi];
   if (ctx->precision == CLV_FP32) launch_init<float>(b, ctx->stream, local, n0, var);
   else launch_init<double>(b, ctx->stream, local, n0, var);
+  b.primed = false;
   HIPCHK(ctx, hipGetLastError());
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   return CLV_OK;
 }
 
 static int step_bucket(clv_ctx *ctx, Bucket &b, int eval_every) {
-  row_kernel(ctx, b, 0, 0, b.nch, 0);
-  controller(ctx, b, 0, b.nch, 0, 0.0, 0);
-  row_kernel(ctx, b, 1, 0, b.nch, 0);
+  if (b.schedule == 2 && b.fused_lds > 0) {
+    if (!b.primed) row_kernel(ctx, b, 0, 0, b.nch, 0);  // partials of the current fields
+    row_kernel(ctx, b, 3, 0, b.nch, 0);
+    b.epoch ^= 1;  // new state and next step's partials live in the other slot
+    b.primed = true;
+  } else {
+    row_kernel(ctx, b, 0, 0, b.nch, 0);
+    controller(ctx, b, 0, b.nch, 0, 0.0, 0);
+    row_kernel(ctx, b, 1, 0, b.nch, 0);
+    b.primed = false;
+  }
   const bool ev = bucket_needs_eval(b, eval_every);
   for (auto &s : b.host_steps) s += 1;
   if (ev) {
@@ -986,6 +1189,7 @@ int clv_apply_update(clv_ctx *ctx, int32_t chain, double dt) {
   controller(ctx, b, local, 1, 2, dt, 0);
   row_kernel(ctx, b, 1, local, 1, 0);
   b.host_steps[local] += 1;
+  b.primed = false;
   HIPCHK(ctx, hipGetLastError());
   return CLV_OK;
 }
@@ -1003,7 +1207,7 @@ int clv_propagate_plain(clv_ctx *ctx, int32_t chain, int64_t nsteps, int32_t eva
     // drift + max/mean on the device, blocking copy to the host
     row_kernel(ctx, b, 0, local, 1, 0);
     controller(ctx, b, local, 1, 1, 0.0, 0);
-    hipError_t e = hipMemcpyAsync(pinned, b.d_st + local, sizeof(ChainState),
+    hipError_t e = hipMemcpyAsync(pinned, b.d_st + (b.epoch & 1) * b.nch + local, sizeof(ChainState),
                                   hipMemcpyDeviceToHost, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) {
@@ -1015,6 +1219,7 @@ int clv_propagate_plain(clv_ctx *ctx, int32_t chain, int64_t nsteps, int32_t eva
     const double dt = clv_compute_new_dt(pinned->mean_drift, pinned->max_drift, &p, &clamp);
     controller(ctx, b, local, 1, 2, dt, clamp);
     row_kernel(ctx, b, 1, local, 1, 0);
+    b.primed = false;
     const bool ev = eval_every > 0 && (b.host_steps[local] % eval_every == 0);
     b.host_steps[local] += 1;
     if (ev) {
@@ -1150,7 +1355,13 @@ int clv_autotune(clv_ctx *ctx, const char *cache_path) {
 int clv_set_schedule(clv_ctx *ctx, int32_t schedule) {
   if (!ctx) return CLV_ERR_ARG;
   if (schedule < 0 || schedule > 2) return fail(ctx, CLV_ERR_ARG, "clv_set_schedule: bad schedule");
-  for (Bucket &b : ctx->buckets) b.schedule = schedule == 0 ? 1 : schedule;
+  for (Bucket &b : ctx->buckets) {
+    if (schedule == 2 && b.fused_lds == 0)
+      return fail(ctx, CLV_ERR_ARG, "clv_set_schedule: the fused schedule does not fit LDS for shape " +
+                                        std::to_string(b.X) + "x" + std::to_string(b.R) + "x" +
+                                        std::to_string(b.Nt) + " at this precision");
+  }
+  for (Bucket &b : ctx->buckets) b.schedule = (schedule == 0) ? (b.fused_lds ? 2 : 1) : schedule;
   return CLV_OK;
 }
 

@@ -17,9 +17,11 @@ struct BucketArgs {
   C2<T> *buf0_psi, *buf0_psib;  // ping-pong buffers; per-chain parity selects
   C2<T> *buf1_psi, *buf1_psib;
   const ChainParams *prm;
-  ChainState *st;
+  ChainState *st;               // controller state read by this launch
+  ChainState *st_next;          // written by the fused kernel (== st otherwise)
   const C2<T> *tw;              // tw[m] = exp(-2 pi i m / X)
-  double *partials;             // [chain][Nt][ntiles][2] (max, sum)
+  double *part_rd;              // [chain][Nt][ntiles][2] (max, sum) of the current fields
+  double *part_wr;              // partials written by this launch
   double *obs_stage;            // [chain][nchunks][Vs][3]
   double *obs_spart;            // [chain][nchunks][ntiles][5]
   double *obs_F;                // [chain][Vs][3]
@@ -181,22 +183,25 @@ __device__ __forceinline__ C2<T> drift0_site(const Coef<T> &k, const C2<T> pm[4]
   return cscale(k.af, cadd(csub(pm[0], p00), cscale(k.eps, inner)));
 }
 
-// Shared-memory carve-up of the row-transform kernels.
+// Shared-memory carve-up of the row-transform kernels: twiddles, the
+// kinetic multiplier table, `nlines` padded lines, reduction scratch.
 template <typename T, int X>
 struct Smem {
+  using F = RowFFT<T, X>;
   C2<T> *tw;
   T *mult;
-  C2<T> *a, *b;
+  C2<T> *buf;
   double *red;
   __device__ Smem(char *base, int nlines) {
     tw = reinterpret_cast<C2<T> *>(base);
     mult = reinterpret_cast<T *>(tw + X);
-    a = reinterpret_cast<C2<T> *>(mult + X + (X & 1));
-    b = a + nlines * X;
-    red = reinterpret_cast<double *>(b + nlines * X);
+    buf = reinterpret_cast<C2<T> *>(mult + X + (X & 1));
+    red = reinterpret_cast<double *>(buf + nlines * F::PAD);
   }
+  // cell of site x in line l
+  __device__ __forceinline__ C2<T> &at(int l, int x) { return buf[l * F::PAD + F::pos(x)]; }
   static size_t bytes(int nlines) {
-    return sizeof(C2<T>) * X + sizeof(T) * (X + (X & 1)) + 2 * sizeof(C2<T>) * nlines * X +
+    return sizeof(C2<T>) * X + sizeof(T) * (X + (X & 1)) + sizeof(C2<T>) * nlines * F::PAD +
            sizeof(double) * 64;
   }
 };
@@ -210,6 +215,39 @@ __device__ __forceinline__ void load_tables(Smem<T, X> &sm, const C2<T> *tw, dou
     const double p = mom(e, X);
     sm.mult[e] = T(-(p * p) * norm);
   }
+}
+
+// Fixed-order (max, sum) of n per-tile partials, by one full wave: lane l
+// folds tiles l, l+64, ... in order, then a fixed xor tree.
+__device__ __forceinline__ void reduce_partials(const double *pp, int n, int lane, double *m,
+                                                double *s) {
+  double mm = 0.0, ss = 0.0;
+  for (int i = lane; i < n; i += 64) {
+    mm = fmax(mm, pp[2 * i]);
+    ss += pp[2 * i + 1];
+  }
+  *m = wave_max(mm);
+  *s = wave_sum(ss);
+}
+
+// Workgroup (max, sum) of v in a fixed order; valid in thread 0.  Uses red[0..63].
+__device__ __forceinline__ void block_max_sum(double v, double *red, int tid, int nthr, double *m,
+                                              double *s) {
+  double mm = wave_max(v), ss = wave_sum(v);
+  const int w = tid >> 6, nw = (nthr + 63) >> 6;
+  if ((tid & 63) == 0) {
+    red[w] = mm;
+    red[32 + w] = ss;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    for (int q = 1; q < nw; ++q) {
+      mm = fmax(mm, red[q]);
+      ss += red[32 + q];
+    }
+  }
+  *m = mm;
+  *s = ss;
 }
 
 }  // namespace clv

@@ -1,0 +1,106 @@
+"""The C++ host driver (clv_main): run-time Parameters.h / params.txt ingestion,
+the reference's stdout strings and output files (main.cu:53-162,
+SaveResults.cu:52-607), and -- on a GPU -- its Skalars.txt numbers against the
+oracle run through the same main loop."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+DRIVER = os.path.join(ROOT, "cl_var_dt_amd", "driver", "clv_main")
+
+PARAMS_H = """#ifndef PARAMETERS_H_
+#define PARAMETERS_H_
+#define COMPONENTS 4
+#define XSIZE 32
+#define YSIZE 1
+#define ZSIZE 1
+#define TAUSIZE 8
+#define EPS     0.125        //beta=EPS*TAUSIZE
+#define a_x     1.0
+#define G       0.10000
+#define M       0.00000
+#define SIGMA   0.00000
+#define D       0.27735
+#define OMEGAX 0.
+#define OMEGAY 0.
+#define OMEGAZ 0.
+#define Q 0.0
+#define STEPS 120
+#define DT 0.002
+#define SNAPSHOT_TRIGGER 50
+#define EVALUATE_TRIGGER 10
+#define BLOCKSIZE 64
+#endif
+"""
+
+
+def _run(tmp_path, *args, params="mu=-0.5\nP=0\nrun=1\nseed=42\nseed2=7\n", params_h=PARAMS_H):
+    if not os.path.exists(DRIVER):
+        pytest.skip("driver not built")
+    (tmp_path / "Parameters.h").write_text(params_h)
+    (tmp_path / "params.txt").write_text(params)
+    return subprocess.run([DRIVER, *args], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+
+
+def test_dry_run_outputs(tmp_path):
+    r = _run(tmp_path, "--dry-run")
+    assert r.returncode == 0, r.stderr
+    out = r.stdout.splitlines()
+    assert out[0] == 'Directory created successfully: "./DATA/BoseCL_Sim_mu_-0.5_P_0/Run1"'
+    assert out[1] == "Saved parameters"
+    assert out[2] == "Directory for raw data was successfully created"
+    assert out[3] == "Started simulation with parameters"
+    assert "T 1" in out and "lattice 32x1x1x8" in out and "seed 42" in out and "steps 120" in out
+    run1 = tmp_path / "DATA" / "BoseCL_Sim_mu_-0.5_P_0" / "Run1"
+    assert (run1 / "rawdata").is_dir()
+    par = (run1 / "Parameters.txt").read_text().split("\n")
+    assert par[0] == "T\t1" and par[1] == "mu\t-0.5" and par[2] == "g\t0.1"
+    assert "lattice\t32x1x1x8" in par and par[-1] == "seed\t42"
+    # a second run goes to Run2 (SaveResults.cu:59-67)
+    r = _run(tmp_path, "--dry-run")
+    assert '"./DATA/BoseCL_Sim_mu_-0.5_P_0/Run2"' in r.stdout
+
+
+def test_missing_mu_is_an_error(tmp_path):
+    r = _run(tmp_path, "--dry-run", params="P=0\nseed=1\n")
+    assert r.returncode == 1
+    assert "params.txt" in r.stderr
+
+
+@pytest.mark.gpu
+def test_driver_run_matches_oracle(tmp_path, oracle_lib):
+    print("case: driver run", flush=True)
+    r = _run(tmp_path, "0")
+    assert r.returncode == 0, r.stdout + r.stderr
+    lines = r.stdout.splitlines()
+    assert lines[-1].startswith("Finished simulation after computation time of ")
+    saved = [ln for ln in lines if ln.startswith("Saved scalars at time ")]
+    # snapshots at i = 1, 50, 100 and after the loop (main.cu:146-158)
+    assert [int(s.split()[-1]) for s in saved] == [1, 5, 10, 12]
+    base = tmp_path / "DATA" / "BoseCL_Sim_mu_-0.5_P_0"
+    sk = [ln.split("\t") for ln in (base / "Skalars.txt").read_text().splitlines()]
+    assert len(sk) == 4 and all(len(row) == 8 for row in sk)
+    fz = (base / "Run1" / "F_z.txt").read_text().splitlines()
+    assert len(fz) == 4 and len(fz[0].rstrip("\t").split("\t")) == 32
+
+    # the same main loop on the oracle, from the reference's zero start
+    from cl_var_dt_amd import Shape, default_params
+    p = default_params(mu=-0.5, P=0.0, G=0.1, Q=0.0, eps=0.125, a_x=1.0, DT=0.002, seed=42)
+    o = oracle_lib.Oracle(Shape(32, 1, 1, 8), p, nthreads=2)
+    z = np.zeros((4, 8, 32, 1, 1), complex)
+    o.set_fields(z, z)
+    rows = []
+    for i in range(120):
+        if (i % 50 == 0 and i not in (0, 119)) or i == 1:
+            obs, _ = o.observables()
+            rows.append([obs.n_tot, *obs.n_comp])
+        o.propagate(1, 0)
+        if i % 10 == 0:
+            o.evaluate()
+    obs, _ = o.observables()
+    rows.append([obs.n_tot, *obs.n_comp])
+    got = np.array([[float(v) for v in row[:5]] for row in sk])
+    assert np.allclose(got, np.array(rows), atol=2e-6)
