@@ -75,6 +75,32 @@ class Observables(C.Structure):
     ]
 
 
+class Request(C.Structure):
+    """clv_request: one Langevin run submitted to a server."""
+
+    _fields_ = [("shape", Shape), ("params", Params), ("nsteps", C.c_int64),
+                ("eval_every", C.c_int32), ("reserved", C.c_int32), ("eval_from", C.c_int64),
+                ("n0", C.c_double), ("var", C.c_double)]
+
+
+class Result(C.Structure):
+    _fields_ = [("id", C.c_int64), ("status", C.c_int32), ("reserved", C.c_int32),
+                ("obs", Observables), ("stats", StepStats), ("t_submit", C.c_double),
+                ("t_admit", C.c_double), ("t_done", C.c_double)]
+
+
+class ServerInfo(C.Structure):
+    _fields_ = [("submitted", C.c_int64), ("completed", C.c_int64), ("running", C.c_int64),
+                ("queued", C.c_int64), ("failed", C.c_int64), ("bytes_in_use", C.c_int64),
+                ("budget_bytes", C.c_int64), ("pools", C.c_int64), ("site_updates", C.c_double),
+                ("busy_seconds", C.c_double)]
+
+
+CLV_RUN_QUEUED = 0
+CLV_RUN_RUNNING = 1
+CLV_RUN_DONE = 2
+
+
 def default_params(**kw) -> Params:
     """Reference defaults (Parameters.h:13-31, Propagator_without_cutoff.cu:391-402)."""
     p = Params(mu=0.0, P=0.0, G=0.1, c1=0.0, Q=0.0, eps=0.01, a_x=1.0, DT=0.002,
@@ -130,6 +156,14 @@ def _declare(lib):
         "clv_get_schedule": (C.c_int, [vp, i32]),
         "clv_set_profiling": (C.c_int, [vp, i32]),
         "clv_kernel_time": (C.c_int, [vp, i32, P(dbl), P(i64)]),
+        "clv_server_create": (C.c_int, [i32, i32, i64, P(vp)]),
+        "clv_server_destroy": (C.c_int, [vp]),
+        "clv_server_submit": (C.c_int, [vp, P(Request), P(i64)]),
+        "clv_server_poll": (C.c_int, [vp, i64, P(Result)]),
+        "clv_server_wait": (C.c_int, [vp, i64, dbl, P(Result)]),
+        "clv_server_info_get": (C.c_int, [vp, P(ServerInfo)]),
+        "clv_server_clock": (dbl, []),
+        "clv_server_error": (C.c_char_p, [vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -148,6 +182,8 @@ def exported_symbols() -> list[str]:
         "clv_get_step_stats", "clv_synchronize", "clv_evaluate", "clv_read_observables",
         "clv_reset_observables", "clv_autotune", "clv_set_schedule", "clv_get_schedule",
         "clv_set_profiling", "clv_kernel_time",
+        "clv_server_create", "clv_server_destroy", "clv_server_submit", "clv_server_poll",
+        "clv_server_wait", "clv_server_info_get", "clv_server_clock", "clv_server_error",
     ]
 
 

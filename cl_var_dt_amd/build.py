@@ -30,7 +30,7 @@ def _run(cmd: list[str]) -> None:
 
 
 def build_lib(extra: list[str] | None = None) -> str:
-    srcs = [os.path.join(CSRC, "clv_engine.hip")]
+    srcs = [os.path.join(CSRC, "clv_engine.hip"), os.path.join(CSRC, "clv_server.hip")]
     tmp = LIB + ".tmp"
     _run([HIPCC, *HIP_FLAGS, *(extra or []), "-I", os.path.join(ROOT, "include"),
           "-o", tmp, *srcs])

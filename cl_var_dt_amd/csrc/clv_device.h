@@ -36,7 +36,9 @@ __device__ __forceinline__ C2<T> cmul_si(C2<T> a) { return {T(-S) * a.y, T(S) * 
 struct ChainParams {
   double mu, P, G, c1, Q, eps, a_x, DT, dt_min, dt_max, alpha, factor;
   uint64_t seed;
-  int32_t q3_mode, noise_on, active, pad;
+  int32_t q3_mode, noise_on, active;
+  int32_t eval_every;  // per-chain observable cadence (serving); 0 never
+  int64_t eval_from;   // first step index a per-chain cadence may evaluate
 };
 
 // Per-chain controller state, device resident.

@@ -23,9 +23,11 @@
 #include <vector>
 
 #include "../../include/clv.h"
+#include "clv_internal.h"
 #include "clv_kernels.h"
 
 using namespace clv;
+using namespace clvi;
 
 // ============================================================== kernels ====
 
@@ -383,7 +385,7 @@ __global__ __launch_bounds__(256) void k_obs(BucketArgs<T> A) {
   const ChainParams &P = A.prm[chain];
   if (!P.active) return;
   const ChainState &st = A.st[chain];
-  if (A.eval_every > 0 && (st.cur_step % A.eval_every) != 0) return;
+  if (!evaluates(A, P, st)) return;
   const int tile = blockIdx.x, chunk = blockIdx.y;
   const int tid = threadIdx.x, nthr = blockDim.x;
   const int j = tid / X, x = tid - (tid / X) * X;
@@ -475,7 +477,7 @@ __global__ __launch_bounds__(256) void k_obs_finish(BucketArgs<T> A) {
   const ChainParams &P = A.prm[chain];
   if (!P.active) return;
   const ChainState &st = A.st[chain];
-  if (A.eval_every > 0 && (st.cur_step % A.eval_every) != 0) return;
+  if (!evaluates(A, P, st)) return;
   const int64_t vs = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (vs < A.Vs) {
     for (int a = 0; a < 3; ++a) {
@@ -530,100 +532,11 @@ __global__ __launch_bounds__(256) void k_init(BucketArgs<T> A, double n0, double
 }
 
 // ========================================================== host side =====
+// ========================================================== host side =====
 namespace {
 
-const char *kVersion = "clv 0.1 (gfx950)";
+const char *kVersion = "clv 0.2 (gfx950)";
 thread_local std::string g_create_error = "";
-
-bool supported_x(int X) {
-  switch (X) {
-    case 4: case 8: case 16: case 24: case 32: case 48: case 64: case 96: case 128: case 192:
-    case 256:
-      return true;
-    default:
-      return false;
-  }
-}
-
-struct Bucket {
-  clv_shape shape;
-  int X = 0, R = 0, Nt = 0;
-  int64_t S = 0, Vs = 0;
-  int nch = 0;
-  std::vector<int> ids;  // global chain ids
-  int rows = 1, ntiles = 1, nthr = 64, tc = 1, nchunks = 1;
-  size_t esz = 0;        // bytes per complex element
-  void *psi[2] = {nullptr, nullptr}, *psib[2] = {nullptr, nullptr};
-  ChainParams *d_prm = nullptr;
-  ChainState *d_st = nullptr;
-  void *d_tw = nullptr;
-  double *d_part = nullptr, *d_stage = nullptr, *d_spart = nullptr, *d_F = nullptr,
-         *d_acc = nullptr;
-  int64_t *d_count = nullptr;
-  std::vector<int64_t> host_steps;
-  int schedule = 1;      // 1 two-pass, 2 fused
-  int epoch = 0;         // slot (epoch & 1) of d_st / d_part is current
-  bool primed = false;   // current partials describe the current fields
-  size_t part_slot = 0;  // doubles per partials slot
-  size_t fused_lds = 0;  // LDS bytes of the fused kernel (0: not available)
-};
-
-}  // namespace
-
-struct clv_ctx {
-  int device = 0;
-  int precision = CLV_FP64;
-  hipStream_t stream = nullptr;
-  std::vector<Bucket> buckets;
-  std::vector<std::pair<int, int>> where;  // chain -> (bucket, local)
-  std::vector<clv_params> params;
-  std::vector<void *> allocs;
-  int64_t bytes = 0;
-  std::string err;
-  // kernel timing with HIP events on `stream`
-  bool profiling = false;
-  struct Pending { int kind; hipEvent_t a, b; };
-  std::vector<Pending> pending;
-  std::vector<hipEvent_t> event_pool;
-  double kt_ms[6] = {0, 0, 0, 0, 0, 0};
-  int64_t kt_n[6] = {0, 0, 0, 0, 0, 0};
-};
-
-namespace {
-
-int fail(clv_ctx *ctx, int code, const std::string &msg) {
-  if (ctx) ctx->err = msg;
-  else g_create_error = msg;
-  return code;
-}
-
-#define HIPCHK(ctx, expr)                                                              \
-  do {                                                                                 \
-    hipError_t e_ = (expr);                                                            \
-    if (e_ != hipSuccess)                                                              \
-      return fail(ctx, CLV_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
-  } while (0)
-
-int dalloc(clv_ctx *ctx, void **p, size_t bytes) {
-  if (bytes == 0) bytes = 16;
-  hipError_t e = hipMalloc(p, bytes);
-  if (e != hipSuccess) {
-    *p = nullptr;
-    return fail(ctx, CLV_ERR_NOMEM, std::string("hipMalloc(") + std::to_string(bytes) +
-                                        "): " + hipGetErrorString(e));
-  }
-  ctx->allocs.push_back(*p);
-  ctx->bytes += int64_t(bytes);
-  return CLV_OK;
-}
-
-ChainParams to_device_params(const clv_params &p) {
-  ChainParams d;
-  d.mu = p.mu; d.P = p.P; d.G = p.G; d.c1 = p.c1; d.Q = p.Q; d.eps = p.eps; d.a_x = p.a_x;
-  d.DT = p.DT; d.dt_min = p.dt_min; d.dt_max = p.dt_max; d.alpha = p.alpha; d.factor = p.factor;
-  d.seed = p.seed; d.q3_mode = p.q3_mode; d.noise_on = p.noise_on; d.active = 1; d.pad = 0;
-  return d;
-}
 
 template <typename T>
 BucketArgs<T> make_args(const Bucket &b, int chain0, int eval_every, bool fused = false) {
@@ -659,30 +572,28 @@ BucketArgs<T> make_args(const Bucket &b, int chain0, int eval_every, bool fused 
 }
 
 // ---- launchers, dispatched on X at run time ----
+// which: 0 drift_stats, 1 update, 2 obs, 3 fused
 template <typename T, int X>
 void launch_row_kernels(const Bucket &b, hipStream_t s, int which, int chain0, int nch,
                         int eval_every) {
-  BucketArgs<T> A = make_args<T>(b, chain0, eval_every);
   const dim3 block(b.nthr);
   if (which == 0) {
+    BucketArgs<T> A = make_args<T>(b, chain0, eval_every);
     const size_t sh = Smem<T, X>::bytes(b.rows);
     hipLaunchKernelGGL((k_drift_stats<T, X>), dim3(b.ntiles, b.Nt, nch), block, sh, s, A);
   } else if (which == 1) {
+    BucketArgs<T> A = make_args<T>(b, chain0, eval_every);
     const size_t sh = Smem<T, X>::bytes(8 * b.rows);
     hipLaunchKernelGGL((k_update<T, X>), dim3(b.ntiles, b.Nt, nch), block, sh, s, A);
   } else if (which == 2) {
+    BucketArgs<T> A = make_args<T>(b, chain0, eval_every);
     const size_t sh = Smem<T, X>::bytes(4 * b.rows);
     hipLaunchKernelGGL((k_obs<T, X>), dim3(b.ntiles, b.nchunks, nch), block, sh, s, A);
   } else {
-    BucketArgs<T> F = make_args<T>(b, chain0, eval_every, true);
+    BucketArgs<T> A = make_args<T>(b, chain0, eval_every, true);
     const size_t sh = Smem<T, X>::bytes(9 * b.rows);
-    hipLaunchKernelGGL((k_fused<T, X>), dim3(b.ntiles, nch), block, sh, s, F);
+    hipLaunchKernelGGL((k_fused<T, X>), dim3(b.ntiles, nch), block, sh, s, A);
   }
-}
-
-template <typename T, int X>
-size_t fused_lds_bytes(int rows) {
-  return Smem<T, X>::bytes(9 * rows);
 }
 
 template <typename T>
@@ -766,8 +677,7 @@ void drain_events(clv_ctx *c) {
 // precision-dispatching wrappers
 void row_kernel(clv_ctx *c, const Bucket &b, int which, int chain0, int nch, int ev) {
   static const int kinds[4] = {0, 2, 3, 5};
-  const int kind = kinds[which];
-  timed(c, kind, [&] {
+  timed(c, kinds[which], [&] {
     if (c->precision == CLV_FP32) dispatch_row<float>(b, c->stream, which, chain0, nch, ev);
     else dispatch_row<double>(b, c->stream, which, chain0, nch, ev);
   });
@@ -805,6 +715,78 @@ int pick_rows(int X, int R) {
   return best;
 }
 
+int dalloc(clv_ctx *ctx, Bucket &b, void **p, size_t bytes) {
+  if (bytes == 0) bytes = 16;
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) {
+    *p = nullptr;
+    return fail(ctx, CLV_ERR_NOMEM, std::string("hipMalloc(") + std::to_string(bytes) +
+                                        "): " + hipGetErrorString(e));
+  }
+  b.allocs.push_back(*p);
+  b.bytes += int64_t(bytes);
+  ctx->bytes += int64_t(bytes);
+  return CLV_OK;
+}
+
+int check_chain(clv_ctx *ctx, int chain) {
+  if (!ctx) return CLV_ERR_ARG;
+  if (chain < 0 || chain >= int(ctx->where.size()))
+    return fail(ctx, CLV_ERR_NOTFOUND, "chain index " + std::to_string(chain) + " out of range");
+  return CLV_OK;
+}
+
+// Does any active chain of the bucket evaluate after its next step?
+bool bucket_needs_eval(const Bucket &b, int ev) {
+  if (ev == 0) return false;
+  for (int i = 0; i < b.nch; ++i) {
+    if (!b.host_active[i]) continue;
+    const int e = ev < 0 ? b.host_eval[i] : ev;
+    if (ev < 0 && b.host_steps[i] < b.host_eval_from[i]) continue;
+    if (e > 0 && b.host_steps[i] % e == 0) return true;
+  }
+  return false;
+}
+
+}  // namespace
+
+// ================================================ internal helpers (clvi) ==
+namespace clvi {
+
+int fail(clv_ctx *ctx, int code, const std::string &msg) {
+  if (ctx) ctx->err = msg;
+  else g_create_error = msg;
+  return code;
+}
+
+bool supported_x(int X) {
+  switch (X) {
+    case 4: case 8: case 16: case 24: case 32: case 48: case 64: case 96: case 128: case 192:
+    case 256:
+      return true;
+    default:
+      return false;
+  }
+}
+
+int64_t chain_bytes(const clv_shape &s, int precision) {
+  const int64_t esz = precision == CLV_FP32 ? 8 : 16;
+  const int64_t S = int64_t(s.nx) * s.ny * s.nz * s.nt;
+  const int64_t Vs = int64_t(s.nx) * s.ny * s.nz;
+  // 2 ping-pong copies of psi and psibar + F accumulators and staging
+  return 4 * S * 4 * esz + 3 * Vs * 8 * (1 + s.nt) + 4096;
+}
+
+ChainParams to_device_params(const clv_params &p, int eval_every, int active, int64_t eval_from) {
+  ChainParams d;
+  d.mu = p.mu; d.P = p.P; d.G = p.G; d.c1 = p.c1; d.Q = p.Q; d.eps = p.eps; d.a_x = p.a_x;
+  d.DT = p.DT; d.dt_min = p.dt_min; d.dt_max = p.dt_max; d.alpha = p.alpha; d.factor = p.factor;
+  d.seed = p.seed; d.q3_mode = p.q3_mode; d.noise_on = p.noise_on; d.active = active;
+  d.eval_every = eval_every;
+  d.eval_from = eval_from;
+  return d;
+}
+
 int setup_bucket(clv_ctx *ctx, Bucket &b) {
   const clv_shape &sh = b.shape;
   b.X = sh.nx;
@@ -823,24 +805,24 @@ int setup_bucket(clv_ctx *ctx, Bucket &b) {
   const size_t field = size_t(b.nch) * size_t(b.S) * 4 * b.esz;
   int rc;
   for (int k = 0; k < 2; ++k) {
-    if ((rc = dalloc(ctx, &b.psi[k], field))) return rc;
-    if ((rc = dalloc(ctx, &b.psib[k], field))) return rc;
+    if ((rc = dalloc(ctx, b, &b.psi[k], field))) return rc;
+    if ((rc = dalloc(ctx, b, &b.psib[k], field))) return rc;
   }
-  if ((rc = dalloc(ctx, reinterpret_cast<void **>(&b.d_prm), sizeof(ChainParams) * b.nch))) return rc;
-  if ((rc = dalloc(ctx, reinterpret_cast<void **>(&b.d_st), 2 * sizeof(ChainState) * b.nch))) return rc;
-  if ((rc = dalloc(ctx, &b.d_tw, b.esz * b.X))) return rc;
+  if ((rc = dalloc(ctx, b, reinterpret_cast<void **>(&b.d_prm), sizeof(ChainParams) * b.nch))) return rc;
+  if ((rc = dalloc(ctx, b, reinterpret_cast<void **>(&b.d_st), 2 * sizeof(ChainState) * b.nch))) return rc;
+  if ((rc = dalloc(ctx, b, &b.d_tw, b.esz * b.X))) return rc;
   b.part_slot = size_t(2) * b.nch * b.Nt * b.ntiles;
-  if ((rc = dalloc(ctx, reinterpret_cast<void **>(&b.d_part), sizeof(double) * 2 * b.part_slot)))
+  if ((rc = dalloc(ctx, b, reinterpret_cast<void **>(&b.d_part), sizeof(double) * 2 * b.part_slot)))
     return rc;
-  if ((rc = dalloc(ctx, reinterpret_cast<void **>(&b.d_stage),
+  if ((rc = dalloc(ctx, b, reinterpret_cast<void **>(&b.d_stage),
                    sizeof(double) * 3 * b.nch * b.nchunks * b.Vs)))
     return rc;
-  if ((rc = dalloc(ctx, reinterpret_cast<void **>(&b.d_spart),
+  if ((rc = dalloc(ctx, b, reinterpret_cast<void **>(&b.d_spart),
                    sizeof(double) * 5 * b.nch * b.nchunks * b.ntiles)))
     return rc;
-  if ((rc = dalloc(ctx, reinterpret_cast<void **>(&b.d_F), sizeof(double) * 3 * b.nch * b.Vs))) return rc;
-  if ((rc = dalloc(ctx, reinterpret_cast<void **>(&b.d_acc), sizeof(double) * 5 * b.nch))) return rc;
-  if ((rc = dalloc(ctx, reinterpret_cast<void **>(&b.d_count), sizeof(int64_t) * b.nch))) return rc;
+  if ((rc = dalloc(ctx, b, reinterpret_cast<void **>(&b.d_F), sizeof(double) * 3 * b.nch * b.Vs))) return rc;
+  if ((rc = dalloc(ctx, b, reinterpret_cast<void **>(&b.d_acc), sizeof(double) * 5 * b.nch))) return rc;
+  if ((rc = dalloc(ctx, b, reinterpret_cast<void **>(&b.d_count), sizeof(int64_t) * b.nch))) return rc;
   // twiddles exp(-2 pi i m / X), computed in FP64 on the host
   std::vector<double> tw(2 * b.X);
   for (int m = 0; m < b.X; ++m) {
@@ -855,6 +837,7 @@ int setup_bucket(clv_ctx *ctx, Bucket &b) {
     HIPCHK(ctx, hipMemcpy(b.d_tw, tw.data(), sizeof(double) * tw.size(), hipMemcpyHostToDevice));
   }
   HIPCHK(ctx, hipMemset(b.d_st, 0, 2 * sizeof(ChainState) * b.nch));
+  HIPCHK(ctx, hipMemset(b.d_prm, 0, sizeof(ChainParams) * b.nch));
   HIPCHK(ctx, hipMemset(b.d_F, 0, sizeof(double) * 3 * b.nch * b.Vs));
   HIPCHK(ctx, hipMemset(b.d_acc, 0, sizeof(double) * 5 * b.nch));
   HIPCHK(ctx, hipMemset(b.d_count, 0, sizeof(int64_t) * b.nch));
@@ -863,6 +846,9 @@ int setup_bucket(clv_ctx *ctx, Bucket &b) {
     HIPCHK(ctx, hipMemset(b.psib[k], 0, field));
   }
   b.host_steps.assign(b.nch, 0);
+  b.host_eval.assign(b.nch, 0);
+  b.host_eval_from.assign(b.nch, 0);
+  b.host_active.assign(b.nch, 1);
   const size_t lds = size_t(b.esz) * b.X + size_t(b.esz / 2) * (b.X + (b.X & 1)) +
                      size_t(b.esz) * 9 * b.rows * (b.X + b.X / split_n2(b.X)) + sizeof(double) * 64;
   b.fused_lds = (lds <= 65536) ? lds : 0;
@@ -870,11 +856,32 @@ int setup_bucket(clv_ctx *ctx, Bucket &b) {
   return CLV_OK;
 }
 
-int check_chain(clv_ctx *ctx, int chain) {
-  if (!ctx) return CLV_ERR_ARG;
-  if (chain < 0 || chain >= int(ctx->where.size()))
-    return fail(ctx, CLV_ERR_NOTFOUND, "chain index " + std::to_string(chain) + " out of range");
-  return CLV_OK;
+void free_bucket(clv_ctx *ctx, Bucket &b) {
+  for (void *p : b.allocs) (void)hipFree(p);
+  b.allocs.clear();
+  ctx->bytes -= b.bytes;
+  b.bytes = 0;
+}
+
+void step_bucket(clv_ctx *ctx, Bucket &b, int eval_every) {
+  if (b.schedule == 2 && b.fused_lds > 0) {
+    if (!b.primed) row_kernel(ctx, b, 0, 0, b.nch, 0);  // partials of the current fields
+    row_kernel(ctx, b, 3, 0, b.nch, 0);
+    b.epoch ^= 1;  // new state and next step's partials live in the other slot
+    b.primed = true;
+  } else {
+    row_kernel(ctx, b, 0, 0, b.nch, 0);
+    controller(ctx, b, 0, b.nch, 0, 0.0, 0);
+    row_kernel(ctx, b, 1, 0, b.nch, 0);
+    b.primed = false;
+  }
+  const bool ev = bucket_needs_eval(b, eval_every);
+  for (int i = 0; i < b.nch; ++i)
+    if (b.host_active[i]) b.host_steps[i] += 1;
+  if (ev) {
+    row_kernel(ctx, b, 2, 0, b.nch, eval_every);
+    obs_finish(ctx, b, 0, b.nch, eval_every);
+  }
 }
 
 int read_state(clv_ctx *ctx, const Bucket &b, int local, ChainState *out) {
@@ -884,15 +891,101 @@ int read_state(clv_ctx *ctx, const Bucket &b, int local, ChainState *out) {
   return CLV_OK;
 }
 
-// Does any chain of the bucket evaluate after its next step?
-bool bucket_needs_eval(const Bucket &b, int ev) {
-  if (ev <= 0) return false;
-  for (int64_t s : b.host_steps)
-    if (s % ev == 0) return true;
-  return false;
+void stats_from_state(const ChainState &st, clv_step_stats *out) {
+  out->dt = st.dt;
+  out->max_drift = st.max_drift;
+  out->mean_drift = st.mean_drift;
+  out->t_langevin = st.t_langevin;
+  out->steps = st.steps;
+  out->clamp_low = st.clamp_low;
+  out->clamp_high = st.clamp_high;
+  out->nonfinite = st.nonfinite;
+  out->reserved = 0;
 }
 
-}  // namespace
+// Give slot `local` a fresh chain: parameters, zero controller state and
+// accumulators (both state slots), Gaussian initial fields, active.  All
+// operations are ordered on the context's stream, so a bucket that is being
+// stepped can admit a chain between two of its steps.
+int chain_admit(clv_ctx *ctx, Bucket &b, int local, const clv_params &p, int eval_every,
+                int64_t eval_from, double n0, double var) {
+  ChainParams d = to_device_params(p, eval_every, 1, eval_from);
+  HIPCHK(ctx, hipMemcpyAsync(b.d_prm + local, &d, sizeof(d), hipMemcpyHostToDevice, ctx->stream));
+  ChainState z;
+  std::memset(&z, 0, sizeof(z));
+  for (int slot = 0; slot < 2; ++slot)
+    HIPCHK(ctx, hipMemcpyAsync(b.d_st + slot * b.nch + local, &z, sizeof(z), hipMemcpyHostToDevice,
+                               ctx->stream));
+  HIPCHK(ctx, hipMemsetAsync(b.d_F + 3 * b.Vs * local, 0, sizeof(double) * 3 * b.Vs, ctx->stream));
+  HIPCHK(ctx, hipMemsetAsync(b.d_acc + 5 * local, 0, sizeof(double) * 5, ctx->stream));
+  HIPCHK(ctx, hipMemsetAsync(b.d_count + local, 0, sizeof(int64_t), ctx->stream));
+  if (ctx->precision == CLV_FP32) launch_init<float>(b, ctx->stream, local, n0, var);
+  else launch_init<double>(b, ctx->stream, local, n0, var);
+  HIPCHK(ctx, hipGetLastError());
+  // the host records must stay valid until the copies ran
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  b.host_steps[local] = 0;
+  b.host_eval[local] = eval_every;
+  b.host_eval_from[local] = eval_from;
+  b.host_active[local] = 1;
+  b.primed = false;
+  return CLV_OK;
+}
+
+void chain_evaluate(clv_ctx *ctx, Bucket &b, int local) {
+  row_kernel(ctx, b, 2, local, 1, 0);
+  obs_finish(ctx, b, local, 1, 0);
+}
+
+int chain_set_active(clv_ctx *ctx, Bucket &b, int local, const clv_params &p, int eval_every,
+                     bool active) {
+  ChainParams d = to_device_params(p, eval_every, active ? 1 : 0, b.host_eval_from[local]);
+  HIPCHK(ctx, hipMemcpyAsync(b.d_prm + local, &d, sizeof(d), hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  b.host_active[local] = active ? 1 : 0;
+  return CLV_OK;
+}
+
+int chain_read_observables(clv_ctx *ctx, Bucket &b, int local, clv_observables *out, double *Fx,
+                           double *Fy, double *Fz) {
+  double acc[5];
+  int64_t count = 0;
+  std::vector<double> F(3 * b.Vs);
+  HIPCHK(ctx, hipMemcpyAsync(acc, b.d_acc + 5 * local, sizeof(acc), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipMemcpyAsync(&count, b.d_count + local, sizeof(count), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipMemcpyAsync(F.data(), b.d_F + 3 * b.Vs * local, sizeof(double) * F.size(),
+                             hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  std::memset(out, 0, sizeof(*out));
+  out->count = count;
+  if (count == 0) return CLV_OK;
+  const double Nt = double(b.Nt), cnt = double(count), Vs = double(b.Vs);
+  double tot = 0.0;
+  for (int c = 0; c < 4; ++c) {
+    // real-space sums; the reference's k-space sums are V_s times larger and
+    // write_n_* divides by V_s (Observables_without_cutoff.cu:1527-1547)
+    out->n_comp[c] = acc[c] / Nt / cnt;
+    tot += acc[c];
+  }
+  out->n_tot = tot / Nt / cnt;
+  out->energy = acc[4] / Nt / cnt;
+  double *dst[3] = {Fx, Fy, Fz};
+  double tots[3] = {0, 0, 0};
+  for (int a = 0; a < 3; ++a)
+    for (int64_t s = 0; s < b.Vs; ++s) {
+      const double v = F[3 * s + a] / Nt / cnt;
+      tots[a] += v;
+      if (dst[a]) dst[a][s] = v;
+    }
+  out->Fx_tot = tots[0];
+  out->Fy_tot = tots[1];
+  out->Fz_tot = tots[2];
+  out->density = out->n_tot / Vs;
+  out->Sz = out->Fz_tot / Vs;
+  return CLV_OK;
+}
+
+}  // namespace clvi
 
 // =============================================================== C ABI ====
 extern "C" {
@@ -1011,7 +1104,7 @@ int clv_destroy(clv_ctx *ctx) {
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   drain_events(ctx);
   for (hipEvent_t e : ctx->event_pool) (void)hipEventDestroy(e);
-  for (void *p : ctx->allocs) (void)hipFree(p);
+  for (Bucket &b : ctx->buckets) free_bucket(ctx, b);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return CLV_OK;
@@ -1036,7 +1129,8 @@ int clv_set_params(clv_ctx *ctx, int32_t chain, const clv_params *p) {
   ctx->params[chain] = *p;
   const auto [bi, local] = ctx->where[chain];
   Bucket &b = ctx->buckets[bi];
-  ChainParams d = to_device_params(*p);
+  ChainParams d = to_device_params(*p, b.host_eval[local], b.host_active[local],
+                                   b.host_eval_from[local]);
   HIPCHK(ctx, hipMemcpyAsync(b.d_prm + local, &d, sizeof(d), hipMemcpyHostToDevice, ctx->stream));
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   return CLV_OK;
@@ -1132,27 +1226,6 @@ int clv_init_fields(clv_ctx *ctx, int32_t chain, double n0, double var) {
   return CLV_OK;
 }
 
-static int step_bucket(clv_ctx *ctx, Bucket &b, int eval_every) {
-  if (b.schedule == 2 && b.fused_lds > 0) {
-    if (!b.primed) row_kernel(ctx, b, 0, 0, b.nch, 0);  // partials of the current fields
-    row_kernel(ctx, b, 3, 0, b.nch, 0);
-    b.epoch ^= 1;  // new state and next step's partials live in the other slot
-    b.primed = true;
-  } else {
-    row_kernel(ctx, b, 0, 0, b.nch, 0);
-    controller(ctx, b, 0, b.nch, 0, 0.0, 0);
-    row_kernel(ctx, b, 1, 0, b.nch, 0);
-    b.primed = false;
-  }
-  const bool ev = bucket_needs_eval(b, eval_every);
-  for (auto &s : b.host_steps) s += 1;
-  if (ev) {
-    row_kernel(ctx, b, 2, 0, b.nch, eval_every);
-    obs_finish(ctx, b, 0, b.nch, eval_every);
-  }
-  return CLV_OK;
-}
-
 int clv_propagate(clv_ctx *ctx, int64_t nsteps, int32_t eval_every) {
   if (!ctx) return CLV_ERR_ARG;
   if (nsteps < 0) return fail(ctx, CLV_ERR_ARG, "clv_propagate: nsteps < 0");
@@ -1240,15 +1313,7 @@ int clv_get_step_stats(clv_ctx *ctx, int32_t chain, clv_step_stats *out) {
   const auto [bi, local] = ctx->where[chain];
   ChainState st;
   if ((rc = read_state(ctx, ctx->buckets[bi], local, &st))) return rc;
-  out->dt = st.dt;
-  out->max_drift = st.max_drift;
-  out->mean_drift = st.mean_drift;
-  out->t_langevin = st.t_langevin;
-  out->steps = st.steps;
-  out->clamp_low = st.clamp_low;
-  out->clamp_high = st.clamp_high;
-  out->nonfinite = st.nonfinite;
-  out->reserved = 0;
+  stats_from_state(st, out);
   return CLV_OK;
 }
 
@@ -1287,41 +1352,7 @@ int clv_read_observables(clv_ctx *ctx, int32_t chain, clv_observables *out, doub
   (void)hipSetDevice(ctx->device);
   const auto [bi, local] = ctx->where[chain];
   Bucket &b = ctx->buckets[bi];
-  double acc[5];
-  int64_t count = 0;
-  std::vector<double> F(3 * b.Vs);
-  HIPCHK(ctx, hipMemcpyAsync(acc, b.d_acc + 5 * local, sizeof(acc), hipMemcpyDeviceToHost, ctx->stream));
-  HIPCHK(ctx, hipMemcpyAsync(&count, b.d_count + local, sizeof(count), hipMemcpyDeviceToHost, ctx->stream));
-  HIPCHK(ctx, hipMemcpyAsync(F.data(), b.d_F + 3 * b.Vs * local, sizeof(double) * F.size(),
-                             hipMemcpyDeviceToHost, ctx->stream));
-  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
-  std::memset(out, 0, sizeof(*out));
-  out->count = count;
-  if (count == 0) return CLV_OK;
-  const double Nt = double(b.Nt), cnt = double(count), Vs = double(b.Vs);
-  double tot = 0.0;
-  for (int c = 0; c < 4; ++c) {
-    // real-space sums; the reference's k-space sums are V_s times larger and
-    // write_n_* divides by V_s (Observables_without_cutoff.cu:1527-1547)
-    out->n_comp[c] = acc[c] / Nt / cnt;
-    tot += acc[c];
-  }
-  out->n_tot = tot / Nt / cnt;
-  out->energy = acc[4] / Nt / cnt;
-  double *dst[3] = {Fx, Fy, Fz};
-  double tots[3] = {0, 0, 0};
-  for (int a = 0; a < 3; ++a)
-    for (int64_t s = 0; s < b.Vs; ++s) {
-      const double v = F[3 * s + a] / Nt / cnt;
-      tots[a] += v;
-      if (dst[a]) dst[a][s] = v;
-    }
-  out->Fx_tot = tots[0];
-  out->Fy_tot = tots[1];
-  out->Fz_tot = tots[2];
-  out->density = out->n_tot / Vs;
-  out->Sz = out->Fz_tot / Vs;
-  return CLV_OK;
+  return chain_read_observables(ctx, b, local, out, Fx, Fy, Fz);
 }
 
 int clv_reset_observables(clv_ctx *ctx, int32_t chain) {

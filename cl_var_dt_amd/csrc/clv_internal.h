@@ -1,0 +1,103 @@
+// clv_internal.h -- host-side internals shared by the engine (clv_engine.hip)
+// and the serving loop (clv_server.hip): shape buckets of chains, the context
+// record, and the launch / per-chain helpers.  Not part of the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../include/clv.h"
+#include "clv_device.h"
+
+namespace clvi {
+
+using clv::ChainParams;
+using clv::ChainState;
+
+// A bucket: `nch` chains of one shape in one set of device buffers.  Kernel
+// launches cover a bucket (or a range of its chains).
+struct Bucket {
+  clv_shape shape{};
+  int X = 0, R = 0, Nt = 0;
+  int64_t S = 0, Vs = 0;
+  int nch = 0;
+  std::vector<int> ids;  // global chain ids (context buckets)
+  int rows = 1, ntiles = 1, nthr = 64, tc = 1, nchunks = 1;
+  size_t esz = 0;        // bytes per complex element
+  void *psi[2] = {nullptr, nullptr}, *psib[2] = {nullptr, nullptr};
+  ChainParams *d_prm = nullptr;
+  ChainState *d_st = nullptr;
+  void *d_tw = nullptr;
+  double *d_part = nullptr, *d_stage = nullptr, *d_spart = nullptr, *d_F = nullptr,
+         *d_acc = nullptr;
+  int64_t *d_count = nullptr;
+  std::vector<int64_t> host_steps;  // steps taken per chain (host mirror)
+  std::vector<int> host_eval;       // per-chain evaluation cadence (serving)
+  std::vector<int64_t> host_eval_from;
+  std::vector<char> host_active;    // per-chain active flag (host mirror)
+  int schedule = 1;      // 1 two-pass, 2 fused
+  int epoch = 0;         // slot (epoch & 1) of d_st / d_part is current
+  bool primed = false;   // current partials describe the current fields
+  size_t part_slot = 0;  // doubles per partials slot
+  size_t fused_lds = 0;  // LDS bytes of the fused kernel (0: not available)
+  std::vector<void *> allocs;
+  int64_t bytes = 0;
+};
+
+}  // namespace clvi
+
+struct clv_ctx {
+  int device = 0;
+  int precision = CLV_FP64;
+  hipStream_t stream = nullptr;
+  std::vector<clvi::Bucket> buckets;
+  std::vector<std::pair<int, int>> where;  // chain -> (bucket, local)
+  std::vector<clv_params> params;
+  int64_t bytes = 0;
+  std::string err;
+  // kernel timing with HIP events on `stream`
+  bool profiling = false;
+  struct Pending { int kind; hipEvent_t a, b; };
+  std::vector<Pending> pending;
+  std::vector<hipEvent_t> event_pool;
+  double kt_ms[6] = {0, 0, 0, 0, 0, 0};
+  int64_t kt_n[6] = {0, 0, 0, 0, 0, 0};
+};
+
+#define HIPCHK(ctx, expr)                                                                  \
+  do {                                                                                     \
+    hipError_t e_ = (expr);                                                                \
+    if (e_ != hipSuccess)                                                                  \
+      return clvi::fail(ctx, CLV_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+namespace clvi {
+
+int fail(clv_ctx *ctx, int code, const std::string &msg);
+bool supported_x(int X);
+// per-chain device bytes of a bucket of this shape and precision
+int64_t chain_bytes(const clv_shape &s, int precision);
+int setup_bucket(clv_ctx *ctx, Bucket &b);
+void free_bucket(clv_ctx *ctx, Bucket &b);
+ChainParams to_device_params(const clv_params &p, int eval_every, int active,
+                             int64_t eval_from = 0);
+// One Langevin step of every active chain of the bucket; eval_every < 0
+// evaluates each chain at its own cadence (ChainParams::eval_every).
+void step_bucket(clv_ctx *ctx, Bucket &b, int eval_every);
+int read_state(clv_ctx *ctx, const Bucket &b, int local, ChainState *out);
+// Serving helpers, all ordered on the context's stream.
+int chain_admit(clv_ctx *ctx, Bucket &b, int local, const clv_params &p, int eval_every,
+                int64_t eval_from, double n0, double var);
+// Accumulate the observables of one chain's current fields now.
+void chain_evaluate(clv_ctx *ctx, Bucket &b, int local);
+int chain_set_active(clv_ctx *ctx, Bucket &b, int local, const clv_params &p, int eval_every,
+                     bool active);
+int chain_read_observables(clv_ctx *ctx, Bucket &b, int local, clv_observables *out, double *Fx,
+                           double *Fy, double *Fz);
+void stats_from_state(const ChainState &st, clv_step_stats *out);
+
+}  // namespace clvi

@@ -31,8 +31,18 @@ struct BucketArgs {
   int64_t S, Vs;
   int R, Nt, rows, ntiles, nchunks, tc;
   int chain0;                   // first chain of the launch
-  int eval_every;               // obs kernels: evaluate chains with cur_step % eval_every == 0
+  int eval_every;               // obs kernels: evaluate chains with cur_step % eval_every == 0;
+                                // < 0: each chain's own ChainParams::eval_every; 0: always
 };
+
+// Does this chain evaluate its observables after the step just applied?
+template <typename T>
+__device__ __forceinline__ bool evaluates(const BucketArgs<T> &A, const ChainParams &P,
+                                          const ChainState &st) {
+  const int ev = A.eval_every < 0 ? P.eval_every : A.eval_every;
+  if (A.eval_every < 0 && (ev <= 0 || st.cur_step < P.eval_from)) return false;
+  return ev <= 0 || (st.cur_step % ev) == 0;
+}
 
 // p of compute_kinetic_part (Propagator_without_cutoff.cu:169-176).
 __device__ __forceinline__ double mom(int j, int n) {

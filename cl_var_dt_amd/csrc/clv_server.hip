@@ -1,0 +1,412 @@
+// clv_server.hip -- the serving loop: continuous batching of Langevin runs.
+//
+// A worker thread owns the device.  Runs wait in a FIFO queue; at every tick
+// boundary the worker admits as many as fit into free slots of the pool of
+// their shape, creating a pool when the HBM budget allows (and releasing
+// empty pools of other shapes when it does not).  Each pool is a bucket of
+// equal-shape chain slots with its own HIP stream, so pools of different
+// shapes run concurrently on the GPU.  A tick advances every active slot by
+// the smallest remaining step count (capped), so every run stops exactly
+// after its own nsteps; finished runs are read back, their slots freed, and
+// waiting clients woken.  Nothing a run computes depends on its slot or on
+// its neighbours: noise is keyed by the run's seed and step, reductions are
+// per chain and in a fixed order.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/clv.h"
+#include "clv_internal.h"
+
+using namespace clvi;
+
+namespace {
+
+double now_s() {
+  using namespace std::chrono;
+  return duration<double>(steady_clock::now().time_since_epoch()).count();
+}
+
+struct Run {
+  int64_t id = 0;
+  clv_request req{};
+  clv_result res{};
+  int pool = -1, slot = -1;
+  int64_t remaining = 0;
+};
+
+struct Pool {
+  clv_ctx ctx;                 // one bucket, own stream
+  std::vector<Run *> slot_run;  // run per slot, nullptr = free
+  int nactive = 0;
+  bool alive = false;
+};
+
+bool same_shape(const clv_shape &a, const clv_shape &b) {
+  return a.nx == b.nx && a.ny == b.ny && a.nz == b.nz && a.nt == b.nt;
+}
+
+constexpr int64_t kTickMax = 25;           // steps per tick at most
+constexpr int64_t kPoolTargetBytes = 4LL << 30;
+constexpr int kPoolMaxSlots = 64;
+
+}  // namespace
+
+struct clv_server {
+  int device = 0;
+  int precision = CLV_FP32;
+  int64_t budget = 0;
+  std::mutex mu;
+  std::condition_variable cv_work, cv_done;
+  std::deque<Run *> queue;
+  std::map<int64_t, std::unique_ptr<Run>> runs;  // guarded by mu
+  std::vector<std::unique_ptr<Pool>> pools;
+  int64_t next_id = 1;
+  bool stop = false;
+  std::thread worker;
+  std::string err;
+  clv_server_info info{};
+  int64_t bytes_in_use = 0;
+};
+
+namespace {
+
+int64_t pool_bytes(const Pool &p) { return p.ctx.bytes; }
+
+// Create a pool for `shape` with as many slots as the target size and the
+// remaining budget allow.  Called by the worker only.
+Pool *make_pool(clv_server *s, const clv_shape &shape) {
+  const int64_t per = chain_bytes(shape, s->precision);
+  int64_t room = s->budget - s->bytes_in_use;
+  int cap = int(std::min<int64_t>(kPoolMaxSlots, std::max<int64_t>(1, kPoolTargetBytes / per)));
+  cap = int(std::min<int64_t>(cap, room / per));
+  if (cap < 1) return nullptr;
+  std::unique_ptr<Pool> p(new Pool());
+  p->ctx.device = s->device;
+  p->ctx.precision = s->precision;
+  if (hipStreamCreateWithFlags(&p->ctx.stream, hipStreamNonBlocking) != hipSuccess) return nullptr;
+  p->ctx.buckets.emplace_back();
+  Bucket &b = p->ctx.buckets.back();
+  b.shape = shape;
+  b.nch = cap;
+  if (setup_bucket(&p->ctx, b) != CLV_OK) {
+    free_bucket(&p->ctx, b);
+    (void)hipStreamDestroy(p->ctx.stream);
+    s->err = p->ctx.err;
+    return nullptr;
+  }
+  b.schedule = b.fused_lds ? 2 : 1;
+  b.host_active.assign(b.nch, 0);
+  // slots start inactive: mark every chain inactive on the device
+  clv_params dflt;
+  clv_default_params(&dflt);
+  for (int i = 0; i < b.nch; ++i) (void)chain_set_active(&p->ctx, b, i, dflt, 0, false);
+  p->slot_run.assign(cap, nullptr);
+  p->alive = true;
+  s->bytes_in_use += pool_bytes(*p);
+  s->pools.push_back(std::move(p));
+  return s->pools.back().get();
+}
+
+void free_pool(clv_server *s, Pool &p) {
+  if (!p.alive) return;
+  (void)hipStreamSynchronize(p.ctx.stream);
+  s->bytes_in_use -= pool_bytes(p);
+  for (Bucket &b : p.ctx.buckets) free_bucket(&p.ctx, b);
+  (void)hipStreamDestroy(p.ctx.stream);
+  p.ctx.buckets.clear();
+  p.alive = false;
+}
+
+// Find a free slot for `shape` (existing pool, new pool, or after releasing
+// empty pools of other shapes).  Returns false if it has to wait.
+bool place(clv_server *s, const clv_shape &shape, int *pool, int *slot) {
+  for (size_t k = 0; k < s->pools.size(); ++k) {
+    Pool &p = *s->pools[k];
+    if (!p.alive || !same_shape(p.ctx.buckets[0].shape, shape)) continue;
+    for (size_t i = 0; i < p.slot_run.size(); ++i)
+      if (p.slot_run[i] == nullptr) {
+        *pool = int(k);
+        *slot = int(i);
+        return true;
+      }
+  }
+  Pool *np = make_pool(s, shape);
+  if (!np) {
+    // release idle pools of other shapes, then try once more
+    bool freed = false;
+    for (auto &pp : s->pools)
+      if (pp->alive && pp->nactive == 0 && !same_shape(pp->ctx.buckets[0].shape, shape)) {
+        free_pool(s, *pp);
+        freed = true;
+      }
+    if (freed) np = make_pool(s, shape);
+  }
+  if (!np) return false;
+  for (size_t k = 0; k < s->pools.size(); ++k)
+    if (s->pools[k].get() == np) {
+      *pool = int(k);
+      *slot = 0;
+      return true;
+    }
+  return false;
+}
+
+void finish_run(clv_server *s, Run &r) {
+  Pool &p = *s->pools[r.pool];
+  Bucket &b = p.ctx.buckets[0];
+  clv_observables obs;
+  ChainState st;
+  int rc = chain_read_observables(&p.ctx, b, r.slot, &obs, nullptr, nullptr, nullptr);
+  if (rc == CLV_OK && obs.count == 0) {
+    // a run that reached no evaluation step is evaluated once, on its final fields
+    chain_evaluate(&p.ctx, b, r.slot);
+    rc = chain_read_observables(&p.ctx, b, r.slot, &obs, nullptr, nullptr, nullptr);
+  }
+  if (rc == CLV_OK) rc = read_state(&p.ctx, b, r.slot, &st);
+  if (rc == CLV_OK) rc = chain_set_active(&p.ctx, b, r.slot, r.req.params, 0, false);
+  std::lock_guard<std::mutex> g(s->mu);
+  if (rc == CLV_OK) {
+    r.res.obs = obs;
+    stats_from_state(st, &r.res.stats);
+    r.res.status = CLV_RUN_DONE;
+    s->info.completed += 1;
+  } else {
+    r.res.status = rc;
+    s->info.failed += 1;
+    s->err = p.ctx.err;
+  }
+  r.res.t_done = now_s();
+  p.slot_run[r.slot] = nullptr;
+  p.nactive -= 1;
+  s->info.running -= 1;
+  s->cv_done.notify_all();
+}
+
+void worker_main(clv_server *s) {
+  (void)hipSetDevice(s->device);
+  double busy_since = -1.0;
+  for (;;) {
+    // ---- admission (tick boundary) ----
+    std::vector<Run *> admitted;
+    {
+      std::unique_lock<std::mutex> lk(s->mu);
+      bool any_active = false;
+      for (auto &p : s->pools) any_active |= (p->alive && p->nactive > 0);
+      if (!any_active && s->queue.empty()) {
+        if (busy_since >= 0) {
+          s->info.busy_seconds += now_s() - busy_since;
+          busy_since = -1.0;
+        }
+        s->cv_work.wait(lk, [&] { return s->stop || !s->queue.empty(); });
+      }
+      if (s->stop) break;
+      for (auto it = s->queue.begin(); it != s->queue.end();) {
+        Run &r = **it;
+        const int64_t per = chain_bytes(r.req.shape, s->precision);
+        if (per > s->budget) {
+          r.res.status = CLV_ERR_BUDGET;
+          r.res.t_done = now_s();
+          s->info.failed += 1;
+          s->info.queued -= 1;
+          it = s->queue.erase(it);
+          s->cv_done.notify_all();
+          continue;
+        }
+        int pool = -1, slot = -1;
+        if (!place(s, r.req.shape, &pool, &slot)) {
+          ++it;
+          continue;
+        }
+        r.pool = pool;
+        r.slot = slot;
+        r.remaining = r.req.nsteps;
+        s->pools[pool]->slot_run[slot] = &r;
+        s->pools[pool]->nactive += 1;
+        r.res.status = CLV_RUN_RUNNING;
+        r.res.t_admit = now_s();
+        s->info.queued -= 1;
+        s->info.running += 1;
+        admitted.push_back(&r);
+        it = s->queue.erase(it);
+      }
+      s->info.pools = 0;
+      for (auto &p : s->pools) s->info.pools += p->alive ? 1 : 0;
+      s->info.bytes_in_use = s->bytes_in_use;
+    }
+    for (Run *rp : admitted) {
+      Run &r = *rp;
+      Pool &p = *s->pools[r.pool];
+      const int rc = chain_admit(&p.ctx, p.ctx.buckets[0], r.slot, r.req.params, r.req.eval_every,
+                                 r.req.eval_from, r.req.n0, r.req.var);
+      if (rc != CLV_OK) {
+        std::lock_guard<std::mutex> g(s->mu);
+        r.res.status = rc;
+        r.res.t_done = now_s();
+        s->err = p.ctx.err;
+        p.slot_run[r.slot] = nullptr;
+        p.nactive -= 1;
+        s->info.running -= 1;
+        s->info.failed += 1;
+        s->cv_done.notify_all();
+      } else if (r.remaining == 0) {
+        finish_run(s, r);
+      }
+    }
+    // ---- one tick: every active slot advances k steps ----
+    int64_t k = kTickMax;
+    bool any = false;
+    for (auto &pp : s->pools) {
+      if (!pp->alive || pp->nactive == 0) continue;
+      for (Run *r : pp->slot_run)
+        if (r) {
+          k = std::min(k, r->remaining);
+          any = true;
+        }
+    }
+    if (!any) continue;
+    if (busy_since < 0) busy_since = now_s();
+    for (int64_t n = 0; n < k; ++n)
+      for (auto &pp : s->pools)
+        if (pp->alive && pp->nactive > 0) step_bucket(&pp->ctx, pp->ctx.buckets[0], -1);
+    double su = 0.0;
+    std::vector<Run *> done;
+    for (auto &pp : s->pools) {
+      if (!pp->alive || pp->nactive == 0) continue;
+      (void)hipStreamSynchronize(pp->ctx.stream);
+      su += double(pp->nactive) * double(pp->ctx.buckets[0].S) * double(k);
+      for (Run *r : pp->slot_run)
+        if (r) {
+          r->remaining -= k;
+          if (r->remaining <= 0) done.push_back(r);
+        }
+    }
+    {
+      std::lock_guard<std::mutex> g(s->mu);
+      s->info.site_updates += su;
+    }
+    for (Run *r : done) finish_run(s, *r);
+  }
+  std::lock_guard<std::mutex> g(s->mu);
+  if (busy_since >= 0) s->info.busy_seconds += now_s() - busy_since;
+}
+
+}  // namespace
+
+extern "C" {
+
+double clv_server_clock(void) { return now_s(); }
+
+const char *clv_server_error(const clv_server *srv) { return srv ? srv->err.c_str() : ""; }
+
+int clv_server_create(int32_t device, int32_t precision, int64_t hbm_budget_bytes,
+                      clv_server **out) {
+  if (!out) return CLV_ERR_ARG;
+  *out = nullptr;
+  if (precision != CLV_FP32 && precision != CLV_FP64) return CLV_ERR_ARG;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0 || device < 0 || device >= ndev)
+    return CLV_ERR_HIP;
+  if (hipSetDevice(device) != hipSuccess) return CLV_ERR_HIP;
+  if (hbm_budget_bytes <= 0) {
+    size_t fr = 0, tot = 0;
+    if (hipMemGetInfo(&fr, &tot) != hipSuccess) return CLV_ERR_HIP;
+    hbm_budget_bytes = int64_t(double(fr) * 0.8);
+  }
+  clv_server *s = new clv_server();
+  s->device = device;
+  s->precision = precision;
+  s->budget = hbm_budget_bytes;
+  s->info.budget_bytes = hbm_budget_bytes;
+  s->worker = std::thread(worker_main, s);
+  *out = s;
+  return CLV_OK;
+}
+
+int clv_server_destroy(clv_server *s) {
+  if (!s) return CLV_OK;
+  {
+    std::lock_guard<std::mutex> g(s->mu);
+    s->stop = true;
+  }
+  s->cv_work.notify_all();
+  if (s->worker.joinable()) s->worker.join();
+  (void)hipSetDevice(s->device);
+  for (auto &p : s->pools) free_pool(s, *p);
+  delete s;
+  return CLV_OK;
+}
+
+int clv_server_submit(clv_server *s, const clv_request *req, int64_t *id) {
+  if (!s || !req) return CLV_ERR_ARG;
+  const clv_shape &sh = req->shape;
+  if (sh.nx <= 0 || sh.ny <= 0 || sh.nz <= 0 || sh.nt < 2 || !supported_x(sh.nx) ||
+      req->nsteps < 0 || req->params.kin_dims != 1 || !(req->params.DT > 0.0) ||
+      !(req->params.eps > 0.0) || !(req->params.a_x > 0.0) || req->n0 < 0 || req->var < 0) {
+    std::lock_guard<std::mutex> g(s->mu);
+    s->err = "clv_server_submit: invalid request (shape, nsteps, kin_dims, DT, eps, a_x, n0, var)";
+    return CLV_ERR_ARG;
+  }
+  std::lock_guard<std::mutex> g(s->mu);
+  std::unique_ptr<Run> r(new Run());
+  r->id = s->next_id++;
+  r->req = *req;
+  r->res.id = r->id;
+  r->res.status = CLV_RUN_QUEUED;
+  r->res.t_submit = now_s();
+  Run *rp = r.get();
+  s->runs[rp->id] = std::move(r);
+  s->queue.push_back(rp);
+  s->info.submitted += 1;
+  s->info.queued += 1;
+  if (id) *id = rp->id;
+  s->cv_work.notify_all();
+  return CLV_OK;
+}
+
+int clv_server_poll(clv_server *s, int64_t id, clv_result *out) {
+  if (!s || !out) return CLV_ERR_ARG;
+  std::lock_guard<std::mutex> g(s->mu);
+  auto it = s->runs.find(id);
+  if (it == s->runs.end()) return CLV_ERR_NOTFOUND;
+  *out = it->second->res;
+  return CLV_OK;
+}
+
+int clv_server_wait(clv_server *s, int64_t id, double timeout_s, clv_result *out) {
+  if (!s || !out) return CLV_ERR_ARG;
+  std::unique_lock<std::mutex> lk(s->mu);
+  auto it = s->runs.find(id);
+  if (it == s->runs.end()) return CLV_ERR_NOTFOUND;
+  const auto deadline = std::chrono::steady_clock::now() +
+                        std::chrono::duration_cast<std::chrono::steady_clock::duration>(
+                            std::chrono::duration<double>(timeout_s));
+  Run *r = it->second.get();
+  const bool ok = s->cv_done.wait_until(lk, deadline, [&] {
+    const int st = r->res.status;
+    return st == CLV_RUN_DONE || st < 0;
+  });
+  *out = r->res;
+  return ok ? CLV_OK : CLV_ERR_STATE;
+}
+
+int clv_server_info_get(clv_server *s, clv_server_info *out) {
+  if (!s || !out) return CLV_ERR_ARG;
+  std::lock_guard<std::mutex> g(s->mu);
+  *out = s->info;
+  out->bytes_in_use = s->bytes_in_use;
+  out->budget_bytes = s->budget;
+  return CLV_OK;
+}
+
+}  // extern "C"

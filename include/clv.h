@@ -189,6 +189,61 @@ int clv_get_schedule(const clv_ctx *ctx, int32_t chain);
 int clv_set_profiling(clv_ctx *ctx, int32_t on);
 int clv_kernel_time(clv_ctx *ctx, int32_t kind, double *total_ms, int64_t *launches);
 
+/* ---- serving: continuous batching of Langevin runs -----------------------
+ * No counterpart in the reference, which runs one chain per process and
+ * batches runs as SLURM job arrays polled from Python (mylib.py:511-543,
+ * 257-276).  A server owns a worker thread and device-resident pools of chain
+ * slots, one pool per lattice shape, each stepped on its own HIP stream.
+ * Requests are admitted into free slots as they arrive, by HBM budget; a run
+ * retires exactly after its nsteps steps and its observables are returned.
+ * A run's result depends only on its request, never on its slot or on the
+ * other runs in flight. */
+typedef struct clv_request {
+  clv_shape shape;
+  clv_params params;
+  int64_t nsteps;      /* Langevin steps of the run                          */
+  int32_t eval_every;  /* observable cadence; 0: once, after the last step   */
+  int32_t reserved;
+  int64_t eval_from;   /* first step index that may be evaluated             */
+  double n0, var;      /* initial fields (see clv_init_fields)               */
+} clv_request;
+
+#define CLV_RUN_QUEUED 0
+#define CLV_RUN_RUNNING 1
+#define CLV_RUN_DONE 2
+
+typedef struct clv_result {
+  int64_t id;
+  int32_t status;      /* CLV_RUN_*, or a negative CLV_ERR_* code            */
+  int32_t reserved;
+  clv_observables obs;
+  clv_step_stats stats;
+  double t_submit, t_admit, t_done;  /* clv_server_clock() seconds          */
+} clv_result;
+
+typedef struct clv_server_info {
+  int64_t submitted, completed, running, queued, failed;
+  int64_t bytes_in_use, budget_bytes, pools;
+  double site_updates;   /* site-updates executed so far                    */
+  double busy_seconds;   /* wall time with at least one run in flight        */
+} clv_server_info;
+
+typedef struct clv_server clv_server;
+
+/* hbm_budget_bytes: device memory the pools may hold (0: 80% of free). */
+int clv_server_create(int32_t device, int32_t precision, int64_t hbm_budget_bytes,
+                      clv_server **out);
+int clv_server_destroy(clv_server *srv);
+int clv_server_submit(clv_server *srv, const clv_request *req, int64_t *id);
+/* Non-blocking; out->status tells where the run is. */
+int clv_server_poll(clv_server *srv, int64_t id, clv_result *out);
+/* Blocks until the run is done or timeout_s passes (CLV_ERR_STATE). */
+int clv_server_wait(clv_server *srv, int64_t id, double timeout_s, clv_result *out);
+int clv_server_info_get(clv_server *srv, clv_server_info *out);
+/* The server's monotonic clock, seconds. */
+double clv_server_clock(void);
+const char *clv_server_error(const clv_server *srv);
+
 #ifdef __cplusplus
 }
 #endif
