@@ -11,7 +11,7 @@ namespace clv {
 
 // ---------------------------------------------------------------- complex --
 template <typename T>
-struct C2 {
+struct alignas(2 * sizeof(T)) C2 {
   T x, y;
 };
 
@@ -260,16 +260,25 @@ template <> struct Split<256> { static constexpr int N1 = 16, N2 = 16; };
 // order.  Each level reads and writes the same LDS cells, so one buffer
 // suffices; a line is padded by one cell per N2 so that both access patterns
 // are free of bank conflicts.
+// Cells per line: at least X + X/N2, and congruent to N2 modulo 32 so that
+// the lanes of a first-level task (consecutive n2, consecutive lines) fall in
+// distinct 8-byte LDS bank pairs.
+__host__ __device__ constexpr int row_pad(int X, int N2) {
+  int p = X + X / N2;
+  while (p % 32 != N2 % 32) ++p;
+  return p;
+}
+
 template <typename T, int X>
 struct RowFFT {
   static constexpr int N1 = Split<X>::N1, N2 = Split<X>::N2;
-  static constexpr int PAD = X + X / N2;  // LDS cells per line
+  static constexpr int PAD = row_pad(X, N2);  // LDS cells per line
   __device__ static __forceinline__ int pos(int p) { return p + p / N2; }
 
   // lines l < nl at buf + l*PAD; tw[m] = exp(-2 pi i m / X); mult[k] = m(k).
   // Caller syncs before (data written) and reads after the trailing sync.
-  __device__ static void kinetic(C2<T> *buf, const C2<T> *tw, const T *mult, int nl, int tid,
-                                 int nthr) {
+  __device__ static __forceinline__ void kinetic(C2<T> *buf, const C2<T> *tw, const T *mult,
+                                                 int nl, int tid, int nthr) {
     for (int task = tid; task < nl * N2; task += nthr) {
       const int l = task / N2, n2 = task - (task / N2) * N2;
       C2<T> *L = buf + l * PAD;

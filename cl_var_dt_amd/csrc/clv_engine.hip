@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -29,6 +30,11 @@
 using namespace clv;
 using namespace clvi;
 
+// Waves per SIMD the fused kernel is compiled for (register budget 512/n).
+#ifndef CLV_FUSED_WAVES_PER_EU
+#define CLV_FUSED_WAVES_PER_EU 2
+#endif
+
 // ============================================================== kernels ====
 
 // D_0 at every site of one tau-slice of `rows` rows; writes the tile's
@@ -40,39 +46,49 @@ __global__ __launch_bounds__(256) void k_drift_stats(BucketArgs<T> A) {
   const int chain = A.chain0 + blockIdx.z;
   const ChainParams &P = A.prm[chain];
   if (!P.active) return;
-  const int tile = blockIdx.x, t = blockIdx.y;
+  const int tile = blockIdx.x, t0 = blockIdx.y * A.tcs;
   const int tid = threadIdx.x, nthr = blockDim.x;
   const int j = tid / X, x = tid - (tid / X) * X;
   const bool has_site = j < A.rows;
-  Smem<T, X> sm(smem_raw, A.rows);
+  const int rows = A.rows, TC = A.tcs;
+  Smem<T, X> sm(smem_raw, TC * rows);
   load_tables<T, X>(sm, A.tw, P.a_x, tid, nthr);
   const int par = A.st[chain].parity;  // current buffer
   const C2<T> *psi = (par ? A.buf1_psi : A.buf0_psi) + chain * A.chain_stride;
   const C2<T> *psib = (par ? A.buf1_psib : A.buf0_psib) + chain * A.chain_stride;
-  const int r = tile * A.rows + j;
-  const int tm1 = (t == 0) ? A.Nt - 1 : t - 1;
-  C2<T> pm[4], b0[4], p00 = {T(0), T(0)};
+  const int r = tile * rows + j;
+  // K_0(t-1) for the TC slices of this workgroup in one batch of TC*rows lines
   if (has_site) {
-    load4(psi, (int64_t(tm1) * A.R + r) * X + x, pm);
-    load4(psib, (int64_t(t) * A.R + r) * X + x, b0);
-    p00 = load1(psi, (int64_t(t) * A.R + r) * X + x, 0);
-    sm.at(j, x) = pm[0];
+    for (int q = 0; q < TC; ++q) {
+      const int t = t0 + q;
+      const int tm1 = (t == 0) ? A.Nt - 1 : t - 1;
+      sm.at(q * rows + j, x) = load1(psi, (int64_t(tm1) * A.R + r) * X + x, 0);
+    }
   }
   __syncthreads();
-  RowFFT<T, X>::kinetic(sm.buf, sm.tw, sm.mult, A.rows, tid, nthr);
-  double v = 0.0;
-  if (has_site) {
-    const Coef<T> k = coef<T>(P);
-    const C2<T> d = drift0_site(k, pm, p00, b0, sm.at(j, x));
-    const double dx = double(d.x), dy = double(d.y);
-    v = sqrt(dx * dx + dy * dy);
-  }
-  double m, s;
-  block_max_sum(v, sm.red, tid, nthr, &m, &s);
-  if (tid == 0) {
-    double *o = A.part_wr + ((int64_t(chain) * A.Nt + t) * A.ntiles + tile) * 2;
-    o[0] = m;
-    o[1] = s;
+  RowFFT<T, X>::kinetic(sm.buf, sm.tw, sm.mult, TC * rows, tid, nthr);
+  const Coef<T> k = coef<T>(P);
+  for (int q = 0; q < TC; ++q) {
+    const int t = t0 + q;
+    double v = 0.0;
+    if (has_site) {
+      const int tm1 = (t == 0) ? A.Nt - 1 : t - 1;
+      C2<T> pm[4], b0[4];
+      load4(psi, (int64_t(tm1) * A.R + r) * X + x, pm);
+      load4(psib, (int64_t(t) * A.R + r) * X + x, b0);
+      const C2<T> p00 = load1(psi, (int64_t(t) * A.R + r) * X + x, 0);
+      const C2<T> d = drift0_site(k, pm, p00, b0, sm.at(q * rows + j, x));
+      const double dx = double(d.x), dy = double(d.y);
+      v = sqrt(dx * dx + dy * dy);
+    }
+    double m, s;
+    block_max_sum(v, sm.red, tid, nthr, &m, &s);
+    if (tid == 0) {
+      double *o = A.part_wr + ((int64_t(chain) * A.Nt + t) * A.ntiles + tile) * 2;
+      o[0] = m;
+      o[1] = s;
+    }
+    __syncthreads();  // sm.red is reused by the next slice
   }
 }
 
@@ -208,7 +224,7 @@ __global__ __launch_bounds__(256) void k_update(BucketArgs<T> A) {
 // as the wrap-around neighbour of tau = Nt-1.  Reads and writes the state once
 // per step; results are bit-identical to the two-pass schedule.
 template <typename T, int X>
-__global__ __launch_bounds__(256) void k_fused(BucketArgs<T> A) {
+__global__ __launch_bounds__(256, CLV_FUSED_WAVES_PER_EU) void k_fused(BucketArgs<T> A) {
   extern __shared__ __align__(16) char smem_raw[];
   const int chain = A.chain0 + blockIdx.y;
   const int tile = blockIdx.x;
@@ -260,14 +276,16 @@ __global__ __launch_bounds__(256) void k_fused(BucketArgs<T> A) {
   const Coef<T> k = coef<T>(P);
   const T dtT = T(dt);
   const T sq = T(sqrt(dt));
-  C2<T> pm[4], p0[4], pp[4], b0[4], bp[4], km[4], kp[4];
-  C2<T> nprev[4], n0k[4], nb0k[4];
+  // the new tau = 0 slice waits in LDS until the old one has been read as
+  // the wrap-around neighbour of tau = Nt-1
+  C2<T> *keep = sm.red_end();
+  auto site_at = [&](int t) { return (int64_t(t) * A.R + r) * X + x; };
+  // registers carried from one tau to the next: old psi(t-1), K(t-1), new psi(t-1)
+  C2<T> pm[4], km[4], nprev[4];
 #pragma unroll
   for (int c = 0; c < 4; ++c) nprev[c] = {T(0), T(0)};
   if (has_site) {
-    load4(psi, (int64_t(Nt - 1) * A.R + r) * X + x, pm);
-    load4(psi, int64_t(r) * X + x, p0);
-    load4(psib, int64_t(r) * X + x, b0);
+    load4(psi, site_at(Nt - 1), pm);
 #pragma unroll
     for (int c = 0; c < 4; ++c) sm.at(c * rows + j, x) = pm[c];
   }
@@ -280,26 +298,31 @@ __global__ __launch_bounds__(256) void k_fused(BucketArgs<T> A) {
   for (int t = 0; t < Nt; ++t) {
     const int tp1 = (t == Nt - 1) ? 0 : t + 1;
     if (has_site) {
-      load4(psi, (int64_t(tp1) * A.R + r) * X + x, pp);
-      load4(psib, (int64_t(tp1) * A.R + r) * X + x, bp);
+      C2<T> v[4];
+      load4(psi, site_at(t), v);
 #pragma unroll
-      for (int c = 0; c < 4; ++c) {
-        sm.at(c * rows + j, x) = p0[c];
-        sm.at((4 + c) * rows + j, x) = bp[c];
-      }
+      for (int c = 0; c < 4; ++c) sm.at(c * rows + j, x) = v[c];
+      load4(psib, site_at(tp1), v);
+#pragma unroll
+      for (int c = 0; c < 4; ++c) sm.at((4 + c) * rows + j, x) = v[c];
       sm.at(8 * rows + j, x) = nprev[0];
     }
     __syncthreads();
     RF::kinetic(sm.buf, sm.tw, sm.mult, 9 * rows, tid, nthr);
     double v = 0.0;
     if (has_site) {
-      C2<T> kn[4];
+      C2<T> p0[4], pp[4], b0[4], bp[4], kn[4], kp[4];
 #pragma unroll
       for (int c = 0; c < 4; ++c) {
         kn[c] = sm.at(c * rows + j, x);
         kp[c] = sm.at((4 + c) * rows + j, x);
       }
       const C2<T> k0n = sm.at(8 * rows + j, x);
+      // old neighbours again, from L1/L2 (not held across the transform)
+      load4(psi, site_at(t), p0);
+      load4(psi, site_at(tp1), pp);
+      load4(psib, site_at(t), b0);
+      load4(psib, site_at(tp1), bp);
       C2<T> D[4], Db[4];
       drift_site(k, pm, p0, pp, b0, bp, km, kp, D, Db);
       T nz[8] = {};
@@ -320,11 +343,11 @@ __global__ __launch_bounds__(256) void k_fused(BucketArgs<T> A) {
       if (t == 0) {
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
-          n0k[c] = o[c];
-          nb0k[c] = ob[c];
+          keep[(c * rows + j) * X + x] = o[c];
+          keep[((4 + c) * rows + j) * X + x] = ob[c];
         }
       } else {
-        const int64_t s0 = (int64_t(t) * A.R + r) * X + x;
+        const int64_t s0 = site_at(t);
         store4(psi, s0, o);
         store4(psib, s0, ob);
         const C2<T> d0 = drift0_site(k, nprev, o[0], ob, k0n);
@@ -334,8 +357,6 @@ __global__ __launch_bounds__(256) void k_fused(BucketArgs<T> A) {
 #pragma unroll
       for (int c = 0; c < 4; ++c) {
         pm[c] = p0[c];
-        p0[c] = pp[c];
-        b0[c] = bp[c];
         km[c] = kn[c];
         nprev[c] = o[c];
       }
@@ -348,11 +369,19 @@ __global__ __launch_bounds__(256) void k_fused(BucketArgs<T> A) {
         o[0] = m;
         o[1] = s;
       }
+    } else {
+      __syncthreads();  // LDS lines are rewritten by the next tau
     }
   }
   // tau = 0 of the new fields, and its |D_0| (needs the new psi(Nt-1))
+  C2<T> n0k[4], nb0k[4];
   if (has_site) {
-    const int64_t s0 = int64_t(r) * X + x;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      n0k[c] = keep[(c * rows + j) * X + x];
+      nb0k[c] = keep[((4 + c) * rows + j) * X + x];
+    }
+    const int64_t s0 = site_at(0);
     store4(psi, s0, n0k);
     store4(psib, s0, nb0k);
     sm.at(j, x) = nprev[0];
@@ -566,6 +595,7 @@ BucketArgs<T> make_args(const Bucket &b, int chain0, int eval_every, bool fused 
   A.ntiles = b.ntiles;
   A.nchunks = b.nchunks;
   A.tc = b.tc;
+  A.tcs = b.tcs;
   A.chain0 = chain0;
   A.eval_every = eval_every;
   return A;
@@ -579,8 +609,8 @@ void launch_row_kernels(const Bucket &b, hipStream_t s, int which, int chain0, i
   const dim3 block(b.nthr);
   if (which == 0) {
     BucketArgs<T> A = make_args<T>(b, chain0, eval_every);
-    const size_t sh = Smem<T, X>::bytes(b.rows);
-    hipLaunchKernelGGL((k_drift_stats<T, X>), dim3(b.ntiles, b.Nt, nch), block, sh, s, A);
+    const size_t sh = Smem<T, X>::bytes(b.tcs * b.rows);
+    hipLaunchKernelGGL((k_drift_stats<T, X>), dim3(b.ntiles, b.Nt / b.tcs, nch), block, sh, s, A);
   } else if (which == 1) {
     BucketArgs<T> A = make_args<T>(b, chain0, eval_every);
     const size_t sh = Smem<T, X>::bytes(8 * b.rows);
@@ -591,7 +621,7 @@ void launch_row_kernels(const Bucket &b, hipStream_t s, int which, int chain0, i
     hipLaunchKernelGGL((k_obs<T, X>), dim3(b.ntiles, b.nchunks, nch), block, sh, s, A);
   } else {
     BucketArgs<T> A = make_args<T>(b, chain0, eval_every, true);
-    const size_t sh = Smem<T, X>::bytes(9 * b.rows);
+    const size_t sh = Smem<T, X>::bytes(9 * b.rows) + sizeof(C2<T>) * 8 * b.rows * X;
     hipLaunchKernelGGL((k_fused<T, X>), dim3(b.ntiles, nch), block, sh, s, A);
   }
 }
@@ -707,13 +737,15 @@ int split_n2(int X) {
   }
 }
 
-// rows per workgroup: the largest divisor of R with X*rows <= 256
-int pick_rows(int X, int R) {
+// rows per workgroup: the largest divisor of R with X*rows <= max_sites
+int pick_rows(int X, int R, int max_sites) {
   int best = 1;
-  for (int q = 1; q <= R && q * X <= 256; ++q)
+  for (int q = 1; q <= R && q * X <= max_sites; ++q)
     if (R % q == 0) best = q;
   return best;
 }
+
+int g_tile_sites = 256;  // sites per workgroup tile (CLV_TILE_SITES overrides)
 
 int dalloc(clv_ctx *ctx, Bucket &b, void **p, size_t bytes) {
   if (bytes == 0) bytes = 16;
@@ -795,13 +827,17 @@ int setup_bucket(clv_ctx *ctx, Bucket &b) {
   b.Vs = int64_t(sh.nx) * sh.ny * sh.nz;
   b.S = b.Vs * sh.nt;
   b.esz = (ctx->precision == CLV_FP32) ? 8 : 16;
-  b.rows = pick_rows(b.X, b.R);
+  if (const char *e = std::getenv("CLV_TILE_SITES")) g_tile_sites = std::max(64, std::min(256, std::atoi(e)));
+  b.rows = pick_rows(b.X, b.R, std::max(g_tile_sites, b.X));
   b.ntiles = b.R / b.rows;
   b.nthr = ((b.X * b.rows + 63) / 64) * 64;
   b.tc = 1;
   for (int q = 1; q <= 8; ++q)
     if (b.Nt % q == 0) b.tc = q;
   b.nchunks = b.Nt / b.tc;
+  b.tcs = 1;  // tau slices per drift-stats workgroup: fill the transform batch
+  for (int q = 1; q <= 8 && q * b.rows * b.X <= 1024; ++q)
+    if (b.Nt % q == 0) b.tcs = q;
   const size_t field = size_t(b.nch) * size_t(b.S) * 4 * b.esz;
   int rc;
   for (int k = 0; k < 2; ++k) {
@@ -850,7 +886,8 @@ int setup_bucket(clv_ctx *ctx, Bucket &b) {
   b.host_eval_from.assign(b.nch, 0);
   b.host_active.assign(b.nch, 1);
   const size_t lds = size_t(b.esz) * b.X + size_t(b.esz / 2) * (b.X + (b.X & 1)) +
-                     size_t(b.esz) * 9 * b.rows * (b.X + b.X / split_n2(b.X)) + sizeof(double) * 64;
+                     size_t(b.esz) * 9 * b.rows * row_pad(b.X, split_n2(b.X)) + sizeof(double) * 64 +
+                     size_t(b.esz) * 8 * b.rows * b.X;
   b.fused_lds = (lds <= 65536) ? lds : 0;
   b.schedule = 1;
   return CLV_OK;
@@ -882,6 +919,26 @@ void step_bucket(clv_ctx *ctx, Bucket &b, int eval_every) {
     row_kernel(ctx, b, 2, 0, b.nch, eval_every);
     obs_finish(ctx, b, 0, b.nch, eval_every);
   }
+}
+
+void enqueue_step_gated(clv_ctx *ctx, Bucket &b, int eval_every) {
+  if (b.schedule == 2 && b.fused_lds > 0) {
+    row_kernel(ctx, b, 3, 0, b.nch, 0);
+    b.epoch ^= 1;
+  } else {
+    row_kernel(ctx, b, 0, 0, b.nch, 0);
+    controller(ctx, b, 0, b.nch, 0, 0.0, 0);
+    row_kernel(ctx, b, 1, 0, b.nch, 0);
+  }
+  if (eval_every != 0) {
+    row_kernel(ctx, b, 2, 0, b.nch, eval_every);
+    obs_finish(ctx, b, 0, b.nch, eval_every);
+  }
+}
+
+void drop_graphs(clv_ctx *ctx) {
+  for (auto &g : ctx->graphs) (void)hipGraphExecDestroy(g.exec);
+  ctx->graphs.clear();
 }
 
 int read_state(clv_ctx *ctx, const Bucket &b, int local, ChainState *out) {
@@ -1103,6 +1160,7 @@ int clv_destroy(clv_ctx *ctx) {
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   drain_events(ctx);
+  drop_graphs(ctx);
   for (hipEvent_t e : ctx->event_pool) (void)hipEventDestroy(e);
   for (Bucket &b : ctx->buckets) free_bucket(ctx, b);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -1226,11 +1284,62 @@ int clv_init_fields(clv_ctx *ctx, int32_t chain, double n0, double var) {
   return CLV_OK;
 }
 
+// Captured graph of two steps of every bucket (two steps so that the fused
+// schedule's alternating state slots return to the captured parity).
+static int graph_for(clv_ctx *ctx, int eval_every, hipGraphExec_t *out) {
+  std::string key = std::to_string(eval_every);
+  for (const Bucket &b : ctx->buckets)
+    key += "/" + std::to_string(b.schedule) + ":" + std::to_string(b.epoch & 1);
+  for (auto &g : ctx->graphs)
+    if (g.key == key) {
+      *out = g.exec;
+      return CLV_OK;
+    }
+  std::vector<int> epochs;
+  for (const Bucket &b : ctx->buckets) epochs.push_back(b.epoch);
+  HIPCHK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+  for (int n = 0; n < 2; ++n)
+    for (Bucket &b : ctx->buckets) enqueue_step_gated(ctx, b, eval_every);
+  hipGraph_t graph = nullptr;
+  const hipError_t ec = hipStreamEndCapture(ctx->stream, &graph);
+  for (size_t i = 0; i < ctx->buckets.size(); ++i) ctx->buckets[i].epoch = epochs[i];
+  if (ec != hipSuccess) return fail(ctx, CLV_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ec));
+  hipGraphExec_t exec = nullptr;
+  const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (ei != hipSuccess) return fail(ctx, CLV_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
+  ctx->graphs.push_back({key, exec});
+  *out = exec;
+  return CLV_OK;
+}
+
 int clv_propagate(clv_ctx *ctx, int64_t nsteps, int32_t eval_every) {
   if (!ctx) return CLV_ERR_ARG;
   if (nsteps < 0) return fail(ctx, CLV_ERR_ARG, "clv_propagate: nsteps < 0");
   (void)hipSetDevice(ctx->device);
-  for (int64_t n = 0; n < nsteps; ++n)
+  if (!ctx->use_graphs || ctx->profiling || nsteps < 4) {
+    for (int64_t n = 0; n < nsteps; ++n)
+      for (Bucket &b : ctx->buckets) step_bucket(ctx, b, eval_every);
+    HIPCHK(ctx, hipGetLastError());
+    return CLV_OK;
+  }
+  // fused buckets need the partials of their current fields before replay
+  for (Bucket &b : ctx->buckets)
+    if (b.schedule == 2 && b.fused_lds > 0 && !b.primed) {
+      row_kernel(ctx, b, 0, 0, b.nch, 0);
+      b.primed = true;
+    }
+  hipGraphExec_t exec = nullptr;
+  int rc = graph_for(ctx, eval_every, &exec);
+  if (rc) return rc;
+  const int64_t pairs = nsteps / 2;
+  for (int64_t k = 0; k < pairs; ++k) HIPCHK(ctx, hipGraphLaunch(exec, ctx->stream));
+  for (Bucket &b : ctx->buckets) {
+    for (int i = 0; i < b.nch; ++i)
+      if (b.host_active[i]) b.host_steps[i] += 2 * pairs;
+    if (!(b.schedule == 2 && b.fused_lds > 0)) b.primed = false;
+  }
+  for (int64_t n = 2 * pairs; n < nsteps; ++n)
     for (Bucket &b : ctx->buckets) step_bucket(ctx, b, eval_every);
   HIPCHK(ctx, hipGetLastError());
   return CLV_OK;
@@ -1393,6 +1502,14 @@ int clv_set_schedule(clv_ctx *ctx, int32_t schedule) {
                                         std::to_string(b.Nt) + " at this precision");
   }
   for (Bucket &b : ctx->buckets) b.schedule = (schedule == 0) ? (b.fused_lds ? 2 : 1) : schedule;
+  drop_graphs(ctx);
+  return CLV_OK;
+}
+
+int clv_set_graphs(clv_ctx *ctx, int32_t on) {
+  if (!ctx) return CLV_ERR_ARG;
+  ctx->use_graphs = on != 0;
+  if (!ctx->use_graphs) drop_graphs(ctx);
   return CLV_OK;
 }
 

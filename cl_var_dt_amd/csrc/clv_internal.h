@@ -26,7 +26,7 @@ struct Bucket {
   int64_t S = 0, Vs = 0;
   int nch = 0;
   std::vector<int> ids;  // global chain ids (context buckets)
-  int rows = 1, ntiles = 1, nthr = 64, tc = 1, nchunks = 1;
+  int rows = 1, ntiles = 1, nthr = 64, tc = 1, nchunks = 1, tcs = 1;
   size_t esz = 0;        // bytes per complex element
   void *psi[2] = {nullptr, nullptr}, *psib[2] = {nullptr, nullptr};
   ChainParams *d_prm = nullptr;
@@ -66,6 +66,11 @@ struct clv_ctx {
   std::vector<hipEvent_t> event_pool;
   double kt_ms[6] = {0, 0, 0, 0, 0, 0};
   int64_t kt_n[6] = {0, 0, 0, 0, 0, 0};
+  // captured two-step graphs of clv_propagate, keyed by the eval cadence and
+  // the buckets' schedules and state-slot parities at capture time
+  bool use_graphs = true;
+  struct GraphEntry { std::string key; hipGraphExec_t exec; };
+  std::vector<GraphEntry> graphs;
 };
 
 #define HIPCHK(ctx, expr)                                                                  \
@@ -88,6 +93,10 @@ ChainParams to_device_params(const clv_params &p, int eval_every, int active,
 // One Langevin step of every active chain of the bucket; eval_every < 0
 // evaluates each chain at its own cadence (ChainParams::eval_every).
 void step_bucket(clv_ctx *ctx, Bucket &b, int eval_every);
+// Launches of one step of bucket b with the observable kernels always
+// enqueued and gated on the device (graph capture); no host bookkeeping.
+void enqueue_step_gated(clv_ctx *ctx, Bucket &b, int eval_every);
+void drop_graphs(clv_ctx *ctx);
 int read_state(clv_ctx *ctx, const Bucket &b, int local, ChainState *out);
 // Serving helpers, all ordered on the context's stream.
 int chain_admit(clv_ctx *ctx, Bucket &b, int local, const clv_params &p, int eval_every,

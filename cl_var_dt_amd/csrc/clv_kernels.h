@@ -30,6 +30,7 @@ struct BucketArgs {
   int64_t chain_stride;         // complex elements per chain field (4 S)
   int64_t S, Vs;
   int R, Nt, rows, ntiles, nchunks, tc;
+  int tcs;                      // tau slices per drift-stats workgroup
   int chain0;                   // first chain of the launch
   int eval_every;               // obs kernels: evaluate chains with cur_step % eval_every == 0;
                                 // < 0: each chain's own ChainParams::eval_every; 0: always
@@ -210,6 +211,8 @@ struct Smem {
   }
   // cell of site x in line l
   __device__ __forceinline__ C2<T> &at(int l, int x) { return buf[l * F::PAD + F::pos(x)]; }
+  // first byte after the reduction scratch (extra per-kernel LDS)
+  __device__ __forceinline__ C2<T> *red_end() { return reinterpret_cast<C2<T> *>(red + 64); }
   static size_t bytes(int nlines) {
     return sizeof(C2<T>) * X + sizeof(T) * (X + (X & 1)) + sizeof(C2<T>) * nlines * F::PAD +
            sizeof(double) * 64;

@@ -138,6 +138,9 @@ class Engine:
     def schedule(self, chain: int = 0) -> int:
         return int(lib().clv_get_schedule(self._h, chain))
 
+    def set_graphs(self, on: bool) -> None:
+        self._chk(lib().clv_set_graphs(self._h, 1 if on else 0))
+
     # ---- kernel timing (HIP events on the context's stream) ----
     def set_profiling(self, on: bool) -> None:
         self._chk(lib().clv_set_profiling(self._h, 1 if on else 0))

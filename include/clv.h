@@ -179,6 +179,9 @@ int clv_autotune(clv_ctx *ctx, const char *cache_path);
  * 2 = fused single pass (update + next step's reduction). */
 int clv_set_schedule(clv_ctx *ctx, int32_t schedule);
 int clv_get_schedule(const clv_ctx *ctx, int32_t chain);
+/* clv_propagate replays a captured HIP graph of two steps of every bucket
+ * (on by default); off runs every step eagerly.  Results are identical. */
+int clv_set_graphs(clv_ctx *ctx, int32_t on);
 
 /* ---- kernel timing -----------------------------------------------------
  * With profiling on, every kernel the context launches is bracketed by HIP

@@ -271,3 +271,38 @@ def test_fused_schedule_matches_oracle(oracle_lib):
     print("fused fp32 rel err", rel)
     assert rel <= 1e-4
     eng.close()
+
+
+@pytest.mark.parametrize("sched", [1, 2])
+def test_graph_replay_equals_eager(sched):
+    """clv_propagate's captured two-step graph gives the eager result bit for bit,
+    replay after replay, with eager work and synchronisations in between."""
+    print("case: graph replay", sched, flush=True)
+    shapes = [(32, 8, 1, 8), (16, 4, 2, 4), (64, 2, 1, 16)]
+    engines = []
+    for graphs in (True, False):
+        e = _engine(shapes, CLV_FP32)
+        for i in range(len(shapes)):
+            e.set_params(i, _params(seed=60 + i, mu=0.9))
+            e.init_fields(i, 1.0, 0.1)
+        e.set_schedule(sched)
+        e.set_graphs(graphs)
+        engines.append(e)
+    g, eager = engines
+    for rep in range(3):
+        g.propagate(6, 3)
+        for _ in range(6):
+            eager.propagate(1, 3)
+        g.synchronize()
+        for i in range(len(shapes)):
+            a, _ = g.get_fields(i)
+            b, _ = eager.get_fields(i)
+            assert np.array_equal(a, b), (rep, i)
+            oa, _ = g.observables(i)
+            ob, _ = eager.observables(i)
+            assert oa.count == ob.count and oa.n_tot == ob.n_tot, (rep, i)
+            assert g.stats(i).t_langevin == eager.stats(i).t_langevin
+        g.evaluate(0)
+        eager.evaluate(0)
+    for e in engines:
+        e.close()
