@@ -45,7 +45,9 @@ def run_batched(args, log):
         eng.init_fields(i, 1.0, 0.1)
     if args.schedule:
         eng.set_schedule(args.schedule)
-    log(f"batched: {nch} chains of {X}x{Y}x1x{Nt} FP32, device bytes {eng.device_bytes()/1e9:.2f} GB")
+    else:
+        eng.autotune(None)
+    log(f"batched: schedule {eng.schedule(0)}, {nch} chains of {X}x{Y}x1x{Nt} FP32, device bytes {eng.device_bytes()/1e9:.2f} GB")
     eng.propagate(args.warmup, 10)
     eng.synchronize()
     torch.cuda.synchronize()
@@ -86,6 +88,132 @@ def run_plain(args, log):
     sites = 64 * 64 * 64
     log(f"plain FP64 64x64x1x64: {args.plain_steps} steps in {wall:.3f} s")
     return dict(su=sites * args.plain_steps / wall, ms_per_step=1e3 * wall / args.plain_steps)
+
+
+def run_uneven(args, log):
+    """256 chains of 3-D lattices L^3 x Nt, L in {16, 24, 32}, Nt in {16, 32, 64}
+    drawn from a seed, beta in [0.5, 2], FP32 fields, graph-captured steps."""
+    import random
+
+    import torch
+    from cl_var_dt_amd import CLV_FP32, Shape
+    from cl_var_dt_amd.engine import Engine
+
+    rng = random.Random(3)
+    shapes, betas = [], []
+    for _ in range(args.uneven_chains):
+        L = rng.choice([16, 24, 32])
+        nt = rng.choice([16, 32, 64])
+        shapes.append(Shape(L, L, L, nt))
+        betas.append(rng.uniform(0.5, 2.0))
+    eng = Engine(shapes, precision=CLV_FP32)
+    for i, (s, beta) in enumerate(zip(shapes, betas)):
+        p = _params(5000 + i, mu=1.0, nt=s.nt)
+        p.eps = beta / s.nt
+        eng.set_params(i, p)
+        eng.init_fields(i, 1.0, 0.1)
+    eng.autotune(os.path.join(ROOT, "gpurun_out", "autotune_cache.jsonl") if args.tune_cache else None)
+    eng.propagate(args.warmup, 10)
+    eng.synchronize()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    eng.propagate(args.uneven_steps, 10)
+    eng.synchronize()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    sites = sum(s.sites for s in shapes)
+    sched = sorted({eng.schedule(i) for i in range(len(shapes))})
+    eng.close()
+    log(f"uneven: {len(shapes)} chains, {sites} sites/step, {args.uneven_steps} steps in {wall:.3f} s")
+    return {"value": round(sites * args.uneven_steps / wall, 1), "unit": "site-updates/s",
+            "chains": len(shapes), "sites_per_step": sites, "steps": args.uneven_steps,
+            "ms_per_step": round(1e3 * wall / args.uneven_steps, 4), "schedules": sched,
+            "workload": "256 chains of 3-D L^3 x Nt, L in {16,24,32}, Nt in {16,32,64} (seeded), "
+                        "beta in [0.5,2], FP32 fields, FP64 reductions, hipGraph-captured steps"}
+
+
+def run_serving(args, log):
+    """Poisson stream of Langevin runs (2-D L^2 x Nt, L in {64,128,256}, Nt in
+    {32,64,128}, mu/c0/c1/q drawn from a seed) at ~70% of measured capacity,
+    plus secant mu searches issuing follow-up runs toward density 1.0."""
+    import random
+    import threading
+
+    import numpy as np
+    from cl_var_dt_amd import CLV_FP32, Shape, default_params
+    from cl_var_dt_amd import serving
+
+    rng = random.Random(11)
+
+    def draw():
+        L = rng.choice([64, 128, 256])
+        nt = rng.choice([32, 64, 128])
+        p = default_params(mu=rng.uniform(0.5, 1.5), G=rng.uniform(0.05, 0.15),
+                           c1=rng.uniform(-0.05, 0.05), Q=rng.uniform(0.0, 0.2), P=0.0,
+                           eps=1.0 / nt, DT=1e-3, seed=rng.getrandbits(48))
+        return Shape(L, L, 1, nt), p
+
+    nsteps = args.serve_steps
+    with serving.Server(precision=CLV_FP32) as srv:
+        # capacity: a closed burst
+        burst = [draw() for _ in range(args.serve_burst)]
+        t0 = time.perf_counter()
+        ids = [srv.submit(s, p, nsteps=nsteps, eval_every=10) for s, p in burst]
+        for i in ids:
+            srv.wait(i)
+        cap_wall = time.perf_counter() - t0
+        su_burst = sum(s.sites for s, _ in burst) * nsteps
+        capacity_su = su_burst / cap_wall
+        mean_su = su_burst / len(burst)
+        rate = args.serve_load * capacity_su / mean_su  # requests per second
+        log(f"serving: capacity {capacity_su:.3e} SU/s, Poisson rate {rate:.2f} req/s")
+
+        # secant clients, issuing follow-ups while the stream runs
+        secant_out = []
+
+        def secant_client(k):
+            # small lattice, long enough runs to relax: density(mu) is then smooth
+            shape = Shape(16, 16, 1, 8)
+            base = 900 + k
+
+            def density(mu):
+                return srv.wait(srv.submit(shape, default_params(
+                    mu=mu, G=0.1, c1=-0.05, Q=0.1, eps=1.0 / 8, DT=1e-2, seed=base),
+                    nsteps=args.secant_steps, eval_every=10, eval_from=args.secant_steps // 3,
+                    n0=0.25, var=0.05)).density
+            mu, hist = serving.secant_mu_search(density, 0.3, 0.8, target=1.0, tol=0.02,
+                                                max_iter=5)
+            secant_out.append({"mu": mu, "iterations": len(hist), "last_residual": hist[-1][1]})
+
+        clients = [threading.Thread(target=secant_client, args=(k,)) for k in range(2)]
+        for c in clients:
+            c.start()
+        # open-loop Poisson arrivals
+        stream = []
+        t_start = time.perf_counter()
+        t_next = t_start
+        while time.perf_counter() - t_start < args.serve_seconds:
+            t_next += rng.expovariate(rate)
+            while time.perf_counter() < t_next:
+                time.sleep(min(0.001, max(0.0, t_next - time.perf_counter())))
+            s, p = draw()
+            stream.append((s, srv.submit(s, p, nsteps=nsteps, eval_every=10)))
+        results = [(s, srv.wait(i)) for s, i in stream]
+        for c in clients:
+            c.join()
+        wall = time.perf_counter() - t_start
+        info = srv.info()
+    lat = np.array([r.latency_s for _, r in results])
+    su = sum(s.sites for s, _ in results) * nsteps
+    return {"throughput": round(su / wall, 1), "unit": "site-updates/s",
+            "p50_s": round(float(np.percentile(lat, 50)), 4),
+            "p99_s": round(float(np.percentile(lat, 99)), 4),
+            "requests": len(results), "rate_req_per_s": round(rate, 3),
+            "load_fraction": args.serve_load, "capacity_su_per_s": round(capacity_su, 1),
+            "steps_per_request": nsteps, "pools": int(info.pools),
+            "hbm_bytes_in_use": int(info.bytes_in_use), "secant": secant_out,
+            "workload": "Poisson stream of 2-D runs, L in {64,128,256}, Nt in {32,64,128}, "
+                        "mu/c0/c1/q from a seed, FP32, HBM-budget admission, continuous batching"}
 
 
 def run_cpu_baseline(args, log):
@@ -131,6 +259,16 @@ def main() -> int:
     ap.add_argument("--quiet", action="store_true")
     ap.add_argument("--mode", choices=["all", "batched"], default="all",
                     help="batched: only the batched leg (for profiling runs)")
+    ap.add_argument("--uneven-chains", type=int, default=256)
+    ap.add_argument("--uneven-steps", type=int, default=20)
+    ap.add_argument("--serve-steps", type=int, default=100)
+    ap.add_argument("--serve-burst", type=int, default=24)
+    ap.add_argument("--serve-seconds", type=float, default=15.0)
+    ap.add_argument("--serve-load", type=float, default=0.7)
+    ap.add_argument("--secant-steps", type=int, default=3000)
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--no-uneven", action="store_true")
+    ap.add_argument("--tune-cache", action="store_true")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -165,6 +303,8 @@ def main() -> int:
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
             "avg_launch_ms": round(ms / n, 4),
             "alg_bytes_per_launch": alg_bytes}
+    unev = None if args.no_uneven else run_uneven(args, log)
+    serv = None if args.no_serving else run_serving(args, log)
     cpu = None if args.no_cpu_baseline else run_cpu_baseline(args, log)
     step_ms = 1e3 * b["wall"] / args.steps
     out = {
@@ -193,6 +333,10 @@ def main() -> int:
         "batched_over_plain": round(b["su"] / p["su"], 2),
         "kernel_ms": {k: round(v[0], 3) for k, v in b["kt"].items() if v[1]},
         "roofline": roof,
+        "uneven_batch": unev,
+        "serving": serv,
+        "p50_s": serv["p50_s"] if serv else None,
+        "p99_s": serv["p99_s"] if serv else None,
         "cpu_baseline": cpu,
         "nonfinite_chains": b["nonfinite"],
     }

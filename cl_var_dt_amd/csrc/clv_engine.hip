@@ -1486,10 +1486,146 @@ int clv_reset_observables(clv_ctx *ctx, int32_t chain) {
   return CLV_OK;
 }
 
-int clv_autotune(clv_ctx *ctx, const char *cache_path) {
-  (void)cache_path;
-  if (!ctx) return CLV_ERR_ARG;
+// ---- autotuner: schedule per shape bucket, timed on the live state ----
+namespace {
+
+std::string device_name(int device) {
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) != hipSuccess) return "unknown";
+  return std::string(prop.gcnArchName) + "/" + std::to_string(prop.multiProcessorCount) + "cu";
+}
+
+std::string tune_key(const std::string &dev, const Bucket &b, int precision) {
+  std::ostringstream k;
+  k << dev << "|" << b.shape.nx << "x" << b.shape.ny << "x" << b.shape.nz << "x" << b.shape.nt
+    << "|" << (precision == CLV_FP32 ? "fp32" : "fp64");
+  return k.str();
+}
+
+// cache lines: {"key": "...", "schedule": S, "ms_two_pass": a, "ms_fused": b}
+std::map<std::string, int> read_tune_cache(const char *path) {
+  std::map<std::string, int> out;
+  if (!path) return out;
+  std::ifstream in(path);
+  std::string line;
+  while (std::getline(in, line)) {
+    const size_t k0 = line.find("\"key\": \"");
+    const size_t s0 = line.find("\"schedule\": ");
+    if (k0 == std::string::npos || s0 == std::string::npos) continue;
+    const size_t k1 = line.find('"', k0 + 8);
+    out[line.substr(k0 + 8, k1 - k0 - 8)] = std::atoi(line.c_str() + s0 + 12);
+  }
+  return out;
+}
+
+// Device copy of everything a step changes in a bucket.
+struct Snapshot {
+  std::vector<std::pair<void *, void *>> saved;  // (device location, copy)
+  std::vector<size_t> sizes;
+  std::vector<int64_t> steps;
+  int epoch = 0;
+  bool primed = false;
+};
+
+int take_snapshot(clv_ctx *ctx, Bucket &b, Snapshot &sn) {
+  const size_t field = size_t(b.nch) * size_t(b.S) * 4 * b.esz;
+  std::vector<std::pair<void *, size_t>> parts = {
+      {b.psi[0], field}, {b.psi[1], field}, {b.psib[0], field}, {b.psib[1], field},
+      {b.d_st, 2 * sizeof(ChainState) * b.nch}, {b.d_part, sizeof(double) * 2 * b.part_slot},
+      {b.d_F, sizeof(double) * 3 * b.nch * b.Vs}, {b.d_acc, sizeof(double) * 5 * b.nch},
+      {b.d_count, sizeof(int64_t) * b.nch}};
+  for (auto &pr : parts) {
+    void *copy = nullptr;
+    if (hipMalloc(&copy, pr.second) != hipSuccess) return fail(ctx, CLV_ERR_NOMEM, "autotune: no room for a state snapshot");
+    HIPCHK(ctx, hipMemcpyAsync(copy, pr.first, pr.second, hipMemcpyDeviceToDevice, ctx->stream));
+    sn.saved.push_back({pr.first, copy});
+    sn.sizes.push_back(pr.second);
+  }
+  sn.steps = b.host_steps;
+  sn.epoch = b.epoch;
+  sn.primed = b.primed;
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   return CLV_OK;
+}
+
+int restore_snapshot(clv_ctx *ctx, Bucket &b, Snapshot &sn, bool release) {
+  for (size_t i = 0; i < sn.saved.size(); ++i)
+    HIPCHK(ctx, hipMemcpyAsync(sn.saved[i].first, sn.saved[i].second, sn.sizes[i],
+                               hipMemcpyDeviceToDevice, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  b.host_steps = sn.steps;
+  b.epoch = sn.epoch;
+  b.primed = sn.primed;
+  if (release) {
+    for (auto &pr : sn.saved) (void)hipFree(pr.second);
+    sn.saved.clear();
+  }
+  return CLV_OK;
+}
+
+// ms per step of bucket b under `schedule` (eager steps, no observables)
+int time_schedule(clv_ctx *ctx, Bucket &b, int schedule, int nsteps, double *ms) {
+  b.schedule = schedule;
+  for (int n = 0; n < 2; ++n) step_bucket(ctx, b, 0);
+  hipEvent_t e0, e1;
+  HIPCHK(ctx, hipEventCreate(&e0));
+  HIPCHK(ctx, hipEventCreate(&e1));
+  HIPCHK(ctx, hipEventRecord(e0, ctx->stream));
+  for (int n = 0; n < nsteps; ++n) step_bucket(ctx, b, 0);
+  HIPCHK(ctx, hipEventRecord(e1, ctx->stream));
+  HIPCHK(ctx, hipEventSynchronize(e1));
+  float t = 0.f;
+  HIPCHK(ctx, hipEventElapsedTime(&t, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  *ms = double(t) / nsteps;
+  return CLV_OK;
+}
+
+}  // namespace
+
+int clv_autotune(clv_ctx *ctx, const char *cache_path) {
+  if (!ctx) return CLV_ERR_ARG;
+  (void)hipSetDevice(ctx->device);
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  const std::string dev = device_name(ctx->device);
+  const std::map<std::string, int> cache = read_tune_cache(cache_path);
+  const bool prof = ctx->profiling;
+  ctx->profiling = false;
+  int rc = CLV_OK;
+  for (Bucket &b : ctx->buckets) {
+    const std::string key = tune_key(dev, b, ctx->precision);
+    auto hit = cache.find(key);
+    if (hit != cache.end() && (hit->second == 1 || (hit->second == 2 && b.fused_lds))) {
+      b.schedule = hit->second;
+      continue;
+    }
+    if (!b.fused_lds) {
+      b.schedule = 1;
+      continue;
+    }
+    Snapshot sn;
+    if ((rc = take_snapshot(ctx, b, sn))) break;
+    double ms1 = 0, ms2 = 0;
+    const int n = 6;
+    rc = time_schedule(ctx, b, 1, n, &ms1);
+    if (!rc) rc = restore_snapshot(ctx, b, sn, false);
+    if (!rc) rc = time_schedule(ctx, b, 2, n, &ms2);
+    const int rc2 = restore_snapshot(ctx, b, sn, true);
+    if (rc || rc2) {
+      rc = rc ? rc : rc2;
+      break;
+    }
+    b.schedule = (ms2 < ms1) ? 2 : 1;
+    if (cache_path) {
+      std::ofstream out(cache_path, std::ios::app);
+      out << "{\"key\": \"" << key << "\", \"schedule\": " << b.schedule << ", \"ms_two_pass\": " << ms1
+          << ", \"ms_fused\": " << ms2 << "}\n";
+    }
+  }
+  ctx->profiling = prof;
+  drop_graphs(ctx);
+  return rc;
 }
 
 int clv_set_schedule(clv_ctx *ctx, int32_t schedule) {

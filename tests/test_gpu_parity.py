@@ -306,3 +306,38 @@ def test_graph_replay_equals_eager(sched):
         eager.evaluate(0)
     for e in engines:
         e.close()
+
+
+def test_autotune_keeps_state_and_caches(tmp_path):
+    """The autotuner times the schedules on a snapshot of the live state and
+    restores it; the choice is cached per (device, shape, precision)."""
+    print("case: autotune", flush=True)
+    shapes = [(64, 4, 1, 8), (16, 8, 2, 4)]
+    cache = str(tmp_path / "tune.jsonl")
+    e = _engine(shapes, CLV_FP32)
+    ref = _engine(shapes, CLV_FP32)
+    for eng in (e, ref):
+        for i in range(len(shapes)):
+            eng.set_params(i, _params(seed=80 + i))
+            eng.init_fields(i, 1.0, 0.1)
+        eng.propagate(3, 0)
+    before = [e.get_fields(i) for i in range(len(shapes))]
+    e.autotune(cache)
+    for i in range(len(shapes)):
+        a, b = e.get_fields(i)
+        assert np.array_equal(a, before[i][0]) and np.array_equal(b, before[i][1])
+        assert e.stats(i).steps == 3
+        assert e.schedule(i) in (1, 2)
+    lines = open(cache).read().strip().splitlines()
+    assert len(lines) == 2 and all('"schedule": ' in ln for ln in lines)
+    e.propagate(5, 0)
+    ref.propagate(5, 0)
+    for i in range(len(shapes)):
+        assert np.array_equal(e.get_fields(i)[0], ref.get_fields(i)[0])
+    # a second context reads the cache instead of timing
+    e2 = _engine(shapes, CLV_FP32)
+    e2.autotune(cache)
+    assert [e2.schedule(i) for i in range(2)] == [e.schedule(i) for i in range(2)]
+    assert len(open(cache).read().strip().splitlines()) == 2
+    for eng in (e, ref, e2):
+        eng.close()
