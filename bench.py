@@ -31,6 +31,21 @@ def _params(seed, mu=1.0, nt=32, c1=-0.05):
     return default_params(mu=mu, G=0.1, c1=c1, Q=0.1, P=0.0, eps=1.0 / nt, DT=1e-3, seed=seed)
 
 
+def measured_traffic(kernel: str, schedule: int, lattice: str, chains: int):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
+    summary of this workload (FETCH_SIZE + WRITE_SIZE, calibrated against a
+    1 GiB copy), or None when no summary matches."""
+    path = os.path.join(ROOT, "profiles", f"traffic_batched_{lattice}_fp32.md")
+    if chains != 128 or not os.path.exists(path):
+        return None, None
+    for line in open(path):
+        parts = line.split()
+        if len(parts) == 4 and parts[0] == "traffic_bytes_per_launch" and parts[1] == "k_" + kernel \
+                and parts[2] == f"schedule={schedule}":
+            return int(parts[3]), os.path.relpath(path, ROOT)
+    return None, None
+
+
 def run_batched(args, log):
     import torch
     from cl_var_dt_amd import CLV_FP32, Shape
@@ -63,8 +78,10 @@ def run_batched(args, log):
     eng.set_profiling(False)
     st = eng.stats(0)
     bad = sum(eng.stats(i).nonfinite for i in range(nch))
+    sched = eng.schedule(0)
     eng.close()
-    return dict(sites=sites, wall=wall, su=sites * args.steps / wall, kt=kt, dt0=st.dt, nonfinite=bad)
+    return dict(sites=sites, wall=wall, su=sites * args.steps / wall, kt=kt, dt0=st.dt, nonfinite=bad,
+                schedule=sched)
 
 
 def run_plain(args, log):
@@ -257,8 +274,9 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--quiet", action="store_true")
-    ap.add_argument("--mode", choices=["all", "batched"], default="all",
-                    help="batched: only the batched leg (for profiling runs)")
+    ap.add_argument("--mode", choices=["all", "batched", "copy"], default="all",
+                    help="batched: only the batched leg (for profiling runs); copy: a 1 GiB "
+                         "device-to-device copy, the calibration kernel of the traffic counters")
     ap.add_argument("--uneven-chains", type=int, default=256)
     ap.add_argument("--uneven-steps", type=int, default=20)
     ap.add_argument("--serve-steps", type=int, default=100)
@@ -285,6 +303,15 @@ def main() -> int:
         print("bench.py needs a GPU", file=sys.stderr)
         return 1
 
+    if args.mode == "copy":
+        x = torch.ones(256 * 1024 * 1024, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        torch.cuda.synchronize()
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": x.numel() * 4, "copies": 3}), flush=True)
+        return 0
     b = run_batched(args, log)
     if args.mode == "batched":
         print(json.dumps({"batched_su_per_s": b["su"], "kernel_ms": b["kt"]}), flush=True)
@@ -299,10 +326,11 @@ def main() -> int:
     avg_s = ms / 1e3 / n
     alg_bytes = alg_per_site.get(dom, 0) * b["sites"]
     achieved = alg_bytes / avg_s / 1e9 if alg_bytes else 0.0
+    traffic, tsrc = measured_traffic(dom, b["schedule"], f"{args.lx}x{args.lx}x1x{args.nt}", args.chains)
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
-            "avg_launch_ms": round(ms / n, 4),
-            "alg_bytes_per_launch": alg_bytes}
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dom,
+            "avg_launch_ms": round(ms / n, 4), "alg_bytes_per_launch": alg_bytes,
+            "traffic_source": tsrc, "schedule": b["schedule"]}
     unev = None if args.no_uneven else run_uneven(args, log)
     serv = None if args.no_serving else run_serving(args, log)
     cpu = None if args.no_cpu_baseline else run_cpu_baseline(args, log)

@@ -57,27 +57,33 @@ __global__ __launch_bounds__(256) void k_drift_stats(BucketArgs<T> A) {
   const C2<T> *psi = (par ? A.buf1_psi : A.buf0_psi) + chain * A.chain_stride;
   const C2<T> *psib = (par ? A.buf1_psib : A.buf0_psib) + chain * A.chain_stride;
   const int r = tile * rows + j;
-  // K_0(t-1) for the TC slices of this workgroup in one batch of TC*rows lines
+  // psi(t-1) of the TC slices, each slice read once: its component 0 feeds
+  // the transform batch and the whole site stays in registers for the drift
+  constexpr int TCMAX = 8;
+  C2<T> pm[TCMAX][4];
   if (has_site) {
-    for (int q = 0; q < TC; ++q) {
+#pragma unroll
+    for (int q = 0; q < TCMAX; ++q) {
+      if (q >= TC) break;
       const int t = t0 + q;
       const int tm1 = (t == 0) ? A.Nt - 1 : t - 1;
-      sm.at(q * rows + j, x) = load1(psi, (int64_t(tm1) * A.R + r) * X + x, 0);
+      load4(psi, (int64_t(tm1) * A.R + r) * X + x, pm[q]);
+      sm.at(q * rows + j, x) = pm[q][0];
     }
   }
   __syncthreads();
   RowFFT<T, X>::kinetic(sm.buf, sm.tw, sm.mult, TC * rows, tid, nthr);
   const Coef<T> k = coef<T>(P);
-  for (int q = 0; q < TC; ++q) {
+#pragma unroll
+  for (int q = 0; q < TCMAX; ++q) {
+    if (q >= TC) break;
     const int t = t0 + q;
     double v = 0.0;
     if (has_site) {
-      const int tm1 = (t == 0) ? A.Nt - 1 : t - 1;
-      C2<T> pm[4], b0[4];
-      load4(psi, (int64_t(tm1) * A.R + r) * X + x, pm);
+      C2<T> b0[4];
       load4(psib, (int64_t(t) * A.R + r) * X + x, b0);
-      const C2<T> p00 = load1(psi, (int64_t(t) * A.R + r) * X + x, 0);
-      const C2<T> d = drift0_site(k, pm, p00, b0, sm.at(q * rows + j, x));
+      const C2<T> p00 = (q + 1 < TC) ? pm[q + 1][0] : load1(psi, (int64_t(t) * A.R + r) * X + x, 0);
+      const C2<T> d = drift0_site(k, pm[q], p00, b0, sm.at(q * rows + j, x));
       const double dx = double(d.x), dy = double(d.y);
       v = sqrt(dx * dx + dy * dy);
     }
@@ -836,7 +842,7 @@ int setup_bucket(clv_ctx *ctx, Bucket &b) {
     if (b.Nt % q == 0) b.tc = q;
   b.nchunks = b.Nt / b.tc;
   b.tcs = 1;  // tau slices per drift-stats workgroup: fill the transform batch
-  for (int q = 1; q <= 8 && q * b.rows * b.X <= 1024; ++q)
+  for (int q = 1; q <= 4 && q * b.rows * b.X <= 1024; ++q)
     if (b.Nt % q == 0) b.tcs = q;
   const size_t field = size_t(b.nch) * size_t(b.S) * 4 * b.esz;
   int rc;
