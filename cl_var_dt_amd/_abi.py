@@ -147,6 +147,7 @@ def _declare(lib):
         "clv_drift_stats": (C.c_int, [vp, i32, P(dbl), P(dbl)]),
         "clv_apply_update": (C.c_int, [vp, i32, dbl]),
         "clv_get_step_stats": (C.c_int, [vp, i32, P(StepStats)]),
+        "clv_set_step_stats": (C.c_int, [vp, i32, P(StepStats)]),
         "clv_synchronize": (C.c_int, [vp]),
         "clv_evaluate": (C.c_int, [vp, i32]),
         "clv_read_observables": (C.c_int, [vp, i32, P(Observables), P(dbl), P(dbl), P(dbl)]),
@@ -180,7 +181,7 @@ def exported_symbols() -> list[str]:
         "clv_create", "clv_destroy", "clv_num_chains", "clv_device_bytes",
         "clv_set_params", "clv_set_fields", "clv_get_fields", "clv_init_fields",
         "clv_propagate", "clv_propagate_plain", "clv_drift_stats", "clv_apply_update",
-        "clv_get_step_stats", "clv_synchronize", "clv_evaluate", "clv_read_observables",
+        "clv_get_step_stats", "clv_set_step_stats", "clv_synchronize", "clv_evaluate", "clv_read_observables",
         "clv_reset_observables", "clv_autotune", "clv_set_schedule", "clv_get_schedule",
         "clv_set_profiling", "clv_kernel_time", "clv_set_graphs",
         "clv_server_create", "clv_server_destroy", "clv_server_submit", "clv_server_poll",

@@ -1432,6 +1432,32 @@ int clv_get_step_stats(clv_ctx *ctx, int32_t chain, clv_step_stats *out) {
   return CLV_OK;
 }
 
+int clv_set_step_stats(clv_ctx *ctx, int32_t chain, const clv_step_stats *in) {
+  int rc = check_chain(ctx, chain);
+  if (rc) return rc;
+  if (!in || in->steps < 0) return fail(ctx, CLV_ERR_ARG, "clv_set_step_stats: bad stats");
+  (void)hipSetDevice(ctx->device);
+  const auto [bi, local] = ctx->where[chain];
+  Bucket &b = ctx->buckets[bi];
+  ChainState st;
+  if ((rc = read_state(ctx, b, local, &st))) return rc;
+  st.dt = in->dt;
+  st.max_drift = in->max_drift;
+  st.mean_drift = in->mean_drift;
+  st.t_langevin = in->t_langevin;
+  st.steps = in->steps;
+  st.cur_step = in->steps > 0 ? in->steps - 1 : 0;
+  st.clamp_low = in->clamp_low;
+  st.clamp_high = in->clamp_high;
+  st.nonfinite = in->nonfinite;
+  HIPCHK(ctx, hipMemcpyAsync(b.d_st + (b.epoch & 1) * b.nch + local, &st, sizeof(st),
+                             hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  b.host_steps[local] = in->steps;
+  b.primed = false;
+  return CLV_OK;
+}
+
 int clv_synchronize(clv_ctx *ctx) {
   if (!ctx) return CLV_ERR_ARG;
   (void)hipSetDevice(ctx->device);

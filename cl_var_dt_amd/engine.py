@@ -109,6 +109,10 @@ class Engine:
         self._chk(lib().clv_get_step_stats(self._h, chain, C.byref(s)))
         return s
 
+    def set_stats(self, chain: int, st: StepStats) -> None:
+        """Restore the controller state (exact resume together with set_fields)."""
+        self._chk(lib().clv_set_step_stats(self._h, chain, C.byref(st)))
+
     def synchronize(self) -> None:
         self._chk(lib().clv_synchronize(self._h))
 

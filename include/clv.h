@@ -160,6 +160,12 @@ int clv_drift_stats(clv_ctx *ctx, int32_t chain, double *max_drift,
 int clv_apply_update(clv_ctx *ctx, int32_t chain, double dt);
 
 int clv_get_step_stats(clv_ctx *ctx, int32_t chain, clv_step_stats *out);
+/* Restore a chain's controller state (steps taken = noise counter, Langevin
+ * time, last dt / max / mean, clamp counters) -- with clv_set_fields this
+ * resumes a run exactly, since the noise is a pure function of (seed, step,
+ * site).  No counterpart in the reference (its save/read helpers,
+ * SaveResults.cu:797-932, are never called and do not save the RNG). */
+int clv_set_step_stats(clv_ctx *ctx, int32_t chain, const clv_step_stats *in);
 int clv_synchronize(clv_ctx *ctx);
 
 /* ---- observables: Observables::evaluate / write_* (Observables_without_cutoff.cu:1155-1524, 1527-1634) */
