@@ -30,9 +30,10 @@
 using namespace clv;
 using namespace clvi;
 
-// Waves per SIMD the fused kernel is compiled for (register budget 512/n).
+// Waves per SIMD the fused kernel is compiled for (register budget 512/n),
+// FP32 fields; the FP64 kernel holds twice the data and is built for 2.
 #ifndef CLV_FUSED_WAVES_PER_EU
-#define CLV_FUSED_WAVES_PER_EU 2
+#define CLV_FUSED_WAVES_PER_EU 3
 #endif
 
 // ============================================================== kernels ====
@@ -52,7 +53,7 @@ __global__ __launch_bounds__(256) void k_drift_stats(BucketArgs<T> A) {
   const bool has_site = j < A.rows;
   const int rows = A.rows, TC = A.tcs;
   Smem<T, X> sm(smem_raw, TC * rows);
-  load_tables<T, X>(sm, A.tw, P.a_x, tid, nthr);
+  load_tables<T, X>(sm, A.tw, A.p2, drift_table_scale(P), tid, nthr);
   const int par = A.st[chain].parity;  // current buffer
   const C2<T> *psi = (par ? A.buf1_psi : A.buf0_psi) + chain * A.chain_stride;
   const C2<T> *psib = (par ? A.buf1_psib : A.buf0_psib) + chain * A.chain_stride;
@@ -158,7 +159,7 @@ __global__ __launch_bounds__(256) void k_update(BucketArgs<T> A) {
   const int j = tid / X, x = tid - (tid / X) * X;
   const bool has_site = j < A.rows;
   Smem<T, X> sm(smem_raw, 8 * A.rows);
-  load_tables<T, X>(sm, A.tw, P.a_x, tid, nthr);
+  load_tables<T, X>(sm, A.tw, A.p2, drift_table_scale(P), tid, nthr);
   const ChainState &st = A.st[chain];
   const int par = st.in_parity;
   const double dt = st.dt;
@@ -230,7 +231,7 @@ __global__ __launch_bounds__(256) void k_update(BucketArgs<T> A) {
 // as the wrap-around neighbour of tau = Nt-1.  Reads and writes the state once
 // per step; results are bit-identical to the two-pass schedule.
 template <typename T, int X>
-__global__ __launch_bounds__(256, CLV_FUSED_WAVES_PER_EU) void k_fused(BucketArgs<T> A) {
+__global__ __launch_bounds__(256, sizeof(T) == 4 ? CLV_FUSED_WAVES_PER_EU : 2) void k_fused(BucketArgs<T> A) {
   extern __shared__ __align__(16) char smem_raw[];
   const int chain = A.chain0 + blockIdx.y;
   const int tile = blockIdx.x;
@@ -244,7 +245,7 @@ __global__ __launch_bounds__(256, CLV_FUSED_WAVES_PER_EU) void k_fused(BucketArg
   const int rows = A.rows, Nt = A.Nt;
   Smem<T, X> sm(smem_raw, 9 * rows);
   using RF = RowFFT<T, X>;
-  load_tables<T, X>(sm, A.tw, P.a_x, tid, nthr);
+  load_tables<T, X>(sm, A.tw, A.p2, drift_table_scale(P), tid, nthr);
   if (tid < 64) {
     const int n = Nt * A.ntiles;
     double m, s;
@@ -286,12 +287,20 @@ __global__ __launch_bounds__(256, CLV_FUSED_WAVES_PER_EU) void k_fused(BucketArg
   // the wrap-around neighbour of tau = Nt-1
   C2<T> *keep = sm.red_end();
   auto site_at = [&](int t) { return (int64_t(t) * A.R + r) * X + x; };
-  // registers carried from one tau to the next: old psi(t-1), K(t-1), new psi(t-1)
-  C2<T> pm[4], km[4], nprev[4];
+  // Registers carried along tau (old = fields of this step, new = updated):
+  //   pm = old psi(t-1), km = K(t-1), nprev = new psi(t-1),
+  //   p0 = old psi(t), b0 = old psibar(t), bp = old psibar(t+1).
+  // Each iteration loads old psi(t+1) (needed after the transform) and
+  // prefetches old psibar(t+2) (the next iteration's transform input), so
+  // every slice is read from memory once and the loads overlap the transform.
+  C2<T> pm[4], km[4], nprev[4], p0[4], b0[4], bp[4];
 #pragma unroll
   for (int c = 0; c < 4; ++c) nprev[c] = {T(0), T(0)};
   if (has_site) {
     load4(psi, site_at(Nt - 1), pm);
+    load4(psi, site_at(0), p0);
+    load4(psib, site_at(0), b0);
+    load4(psib, site_at(1 % Nt), bp);
 #pragma unroll
     for (int c = 0; c < 4; ++c) sm.at(c * rows + j, x) = pm[c];
   }
@@ -301,34 +310,32 @@ __global__ __launch_bounds__(256, CLV_FUSED_WAVES_PER_EU) void k_fused(BucketArg
 #pragma unroll
     for (int c = 0; c < 4; ++c) km[c] = sm.at(c * rows + j, x);
   }
+  __syncthreads();  // the LDS lines are rewritten below
   for (int t = 0; t < Nt; ++t) {
     const int tp1 = (t == Nt - 1) ? 0 : t + 1;
+    C2<T> pp[4], bn[4];
     if (has_site) {
-      C2<T> v[4];
-      load4(psi, site_at(t), v);
+      load4(psi, site_at(tp1), pp);
+      // old psibar(t+2); at t = Nt-2 that is tau = 0, still unwritten (kept in LDS)
+      if (t + 2 <= Nt) load4(psib, site_at(t + 2 == Nt ? 0 : t + 2), bn);
 #pragma unroll
-      for (int c = 0; c < 4; ++c) sm.at(c * rows + j, x) = v[c];
-      load4(psib, site_at(tp1), v);
-#pragma unroll
-      for (int c = 0; c < 4; ++c) sm.at((4 + c) * rows + j, x) = v[c];
+      for (int c = 0; c < 4; ++c) {
+        sm.at(c * rows + j, x) = p0[c];
+        sm.at((4 + c) * rows + j, x) = bp[c];
+      }
       sm.at(8 * rows + j, x) = nprev[0];
     }
     __syncthreads();
     RF::kinetic(sm.buf, sm.tw, sm.mult, 9 * rows, tid, nthr);
     double v = 0.0;
     if (has_site) {
-      C2<T> p0[4], pp[4], b0[4], bp[4], kn[4], kp[4];
+      C2<T> kn[4], kp[4];
 #pragma unroll
       for (int c = 0; c < 4; ++c) {
         kn[c] = sm.at(c * rows + j, x);
         kp[c] = sm.at((4 + c) * rows + j, x);
       }
       const C2<T> k0n = sm.at(8 * rows + j, x);
-      // old neighbours again, from L1/L2 (not held across the transform)
-      load4(psi, site_at(t), p0);
-      load4(psi, site_at(tp1), pp);
-      load4(psib, site_at(t), b0);
-      load4(psib, site_at(tp1), bp);
       C2<T> D[4], Db[4];
       drift_site(k, pm, p0, pp, b0, bp, km, kp, D, Db);
       T nz[8] = {};
@@ -363,6 +370,9 @@ __global__ __launch_bounds__(256, CLV_FUSED_WAVES_PER_EU) void k_fused(BucketArg
 #pragma unroll
       for (int c = 0; c < 4; ++c) {
         pm[c] = p0[c];
+        p0[c] = pp[c];
+        b0[c] = bp[c];
+        bp[c] = bn[c];
         km[c] = kn[c];
         nprev[c] = o[c];
       }
@@ -426,7 +436,7 @@ __global__ __launch_bounds__(256) void k_obs(BucketArgs<T> A) {
   const int j = tid / X, x = tid - (tid / X) * X;
   const bool has_site = j < A.rows;
   Smem<T, X> sm(smem_raw, 4 * A.rows);
-  load_tables<T, X>(sm, A.tw, P.a_x, tid, nthr);
+  load_tables<T, X>(sm, A.tw, A.p2, 1.0 / P.a_x, tid, nthr);
   const int par = st.parity;
   const C2<T> *psi = (par ? A.buf1_psi : A.buf0_psi) + chain * A.chain_stride;
   const C2<T> *psib = (par ? A.buf1_psib : A.buf0_psib) + chain * A.chain_stride;
@@ -585,6 +595,7 @@ BucketArgs<T> make_args(const Bucket &b, int chain0, int eval_every, bool fused 
   A.st = b.d_st + slot * b.nch;
   A.st_next = fused ? b.d_st + (slot ^ 1) * b.nch : A.st;
   A.tw = static_cast<const C2<T> *>(b.d_tw);
+  A.p2 = b.d_p2;
   A.part_rd = b.d_part + slot * b.part_slot;
   A.part_wr = fused ? b.d_part + (slot ^ 1) * b.part_slot : A.part_rd;
   A.obs_stage = b.d_stage;
@@ -632,22 +643,27 @@ void launch_row_kernels(const Bucket &b, hipStream_t s, int which, int chain0, i
   }
 }
 
+#define CLV_SWITCH_X(X_, CALL)                 \
+  switch (X_) {                                \
+    case 4: CALL(4); break;                    \
+    case 8: CALL(8); break;                    \
+    case 16: CALL(16); break;                  \
+    case 24: CALL(24); break;                  \
+    case 32: CALL(32); break;                  \
+    case 48: CALL(48); break;                  \
+    case 64: CALL(64); break;                  \
+    case 96: CALL(96); break;                  \
+    case 128: CALL(128); break;                \
+    case 192: CALL(192); break;                \
+    case 256: CALL(256); break;                \
+    default: break;                            \
+  }
+
 template <typename T>
 void dispatch_row(const Bucket &b, hipStream_t s, int which, int chain0, int nch, int ev) {
-  switch (b.X) {
-    case 4: launch_row_kernels<T, 4>(b, s, which, chain0, nch, ev); break;
-    case 8: launch_row_kernels<T, 8>(b, s, which, chain0, nch, ev); break;
-    case 16: launch_row_kernels<T, 16>(b, s, which, chain0, nch, ev); break;
-    case 24: launch_row_kernels<T, 24>(b, s, which, chain0, nch, ev); break;
-    case 32: launch_row_kernels<T, 32>(b, s, which, chain0, nch, ev); break;
-    case 48: launch_row_kernels<T, 48>(b, s, which, chain0, nch, ev); break;
-    case 64: launch_row_kernels<T, 64>(b, s, which, chain0, nch, ev); break;
-    case 96: launch_row_kernels<T, 96>(b, s, which, chain0, nch, ev); break;
-    case 128: launch_row_kernels<T, 128>(b, s, which, chain0, nch, ev); break;
-    case 192: launch_row_kernels<T, 192>(b, s, which, chain0, nch, ev); break;
-    case 256: launch_row_kernels<T, 256>(b, s, which, chain0, nch, ev); break;
-    default: break;
-  }
+#define CLV_ROW(XX) launch_row_kernels<T, XX>(b, s, which, chain0, nch, ev)
+  CLV_SWITCH_X(b.X, CLV_ROW)
+#undef CLV_ROW
 }
 
 template <typename T>
@@ -853,6 +869,7 @@ int setup_bucket(clv_ctx *ctx, Bucket &b) {
   if ((rc = dalloc(ctx, b, reinterpret_cast<void **>(&b.d_prm), sizeof(ChainParams) * b.nch))) return rc;
   if ((rc = dalloc(ctx, b, reinterpret_cast<void **>(&b.d_st), 2 * sizeof(ChainState) * b.nch))) return rc;
   if ((rc = dalloc(ctx, b, &b.d_tw, b.esz * b.X))) return rc;
+  if ((rc = dalloc(ctx, b, reinterpret_cast<void **>(&b.d_p2), sizeof(double) * b.X))) return rc;
   b.part_slot = size_t(2) * b.nch * b.Nt * b.ntiles;
   if ((rc = dalloc(ctx, b, reinterpret_cast<void **>(&b.d_part), sizeof(double) * 2 * b.part_slot)))
     return rc;
@@ -872,6 +889,14 @@ int setup_bucket(clv_ctx *ctx, Bucket &b) {
     tw[2 * m] = std::cos(a);
     tw[2 * m + 1] = -std::sin(a);
   }
+  // -p_x(k)^2 / X with p of compute_kinetic_part (Propagator_without_cutoff.cu:169-176)
+  std::vector<double> p2(b.X);
+  for (int k = 0; k < b.X; ++k) {
+    const double p = (k < b.X / 2 || b.X == 1) ? 6.283185307179586 * double(k) / double(b.X)
+                                                : -6.283185307179586 * double(b.X - k) / double(b.X);
+    p2[k] = -(p * p) / double(b.X);
+  }
+  HIPCHK(ctx, hipMemcpy(b.d_p2, p2.data(), sizeof(double) * b.X, hipMemcpyHostToDevice));
   if (ctx->precision == CLV_FP32) {
     std::vector<float> twf(tw.begin(), tw.end());
     HIPCHK(ctx, hipMemcpy(b.d_tw, twf.data(), sizeof(float) * twf.size(), hipMemcpyHostToDevice));
@@ -906,6 +931,11 @@ void free_bucket(clv_ctx *ctx, Bucket &b) {
   b.bytes = 0;
 }
 
+void count_step(Bucket &b, int64_t n) {
+  for (int i = 0; i < b.nch; ++i)
+    if (b.host_active[i]) b.host_steps[i] += n;
+}
+
 void step_bucket(clv_ctx *ctx, Bucket &b, int eval_every) {
   if (b.schedule == 2 && b.fused_lds > 0) {
     if (!b.primed) row_kernel(ctx, b, 0, 0, b.nch, 0);  // partials of the current fields
@@ -919,8 +949,7 @@ void step_bucket(clv_ctx *ctx, Bucket &b, int eval_every) {
     b.primed = false;
   }
   const bool ev = bucket_needs_eval(b, eval_every);
-  for (int i = 0; i < b.nch; ++i)
-    if (b.host_active[i]) b.host_steps[i] += 1;
+  count_step(b, 1);
   if (ev) {
     row_kernel(ctx, b, 2, 0, b.nch, eval_every);
     obs_finish(ctx, b, 0, b.nch, eval_every);
@@ -1341,8 +1370,7 @@ int clv_propagate(clv_ctx *ctx, int64_t nsteps, int32_t eval_every) {
   const int64_t pairs = nsteps / 2;
   for (int64_t k = 0; k < pairs; ++k) HIPCHK(ctx, hipGraphLaunch(exec, ctx->stream));
   for (Bucket &b : ctx->buckets) {
-    for (int i = 0; i < b.nch; ++i)
-      if (b.host_active[i]) b.host_steps[i] += 2 * pairs;
+    count_step(b, 2 * pairs);
     if (!(b.schedule == 2 && b.fused_lds > 0)) b.primed = false;
   }
   for (int64_t n = 2 * pairs; n < nsteps; ++n)
@@ -1535,6 +1563,7 @@ std::string tune_key(const std::string &dev, const Bucket &b, int precision) {
 }
 
 // cache lines: {"key": "...", "schedule": S, "ms_two_pass": a, "ms_fused": b}
+// (0 for a schedule that does not apply)
 std::map<std::string, int> read_tune_cache(const char *path) {
   std::map<std::string, int> out;
   if (!path) return out;
@@ -1614,6 +1643,10 @@ int time_schedule(clv_ctx *ctx, Bucket &b, int schedule, int nsteps, double *ms)
   return CLV_OK;
 }
 
+bool schedule_fits(const Bucket &b, int schedule) {
+  return schedule == 1 || (schedule == 2 && b.fused_lds > 0);
+}
+
 }  // namespace
 
 int clv_autotune(clv_ctx *ctx, const char *cache_path) {
@@ -1626,33 +1659,40 @@ int clv_autotune(clv_ctx *ctx, const char *cache_path) {
   ctx->profiling = false;
   int rc = CLV_OK;
   for (Bucket &b : ctx->buckets) {
-    const std::string key = tune_key(dev, b, ctx->precision);
+    const std::string key = tune_key(dev, b, ctx->precision) + "|" + std::to_string(b.nch) + "ch";
     auto hit = cache.find(key);
-    if (hit != cache.end() && (hit->second == 1 || (hit->second == 2 && b.fused_lds))) {
+    if (hit != cache.end() && hit->second >= 1 && hit->second <= 2 && schedule_fits(b, hit->second)) {
       b.schedule = hit->second;
       continue;
     }
-    if (!b.fused_lds) {
+    std::vector<int> cand;
+    for (int sc = 1; sc <= 2; ++sc)
+      if (schedule_fits(b, sc)) cand.push_back(sc);
+    if (cand.size() == 1) {
       b.schedule = 1;
       continue;
     }
     Snapshot sn;
     if ((rc = take_snapshot(ctx, b, sn))) break;
-    double ms1 = 0, ms2 = 0;
-    const int n = 6;
-    rc = time_schedule(ctx, b, 1, n, &ms1);
-    if (!rc) rc = restore_snapshot(ctx, b, sn, false);
-    if (!rc) rc = time_schedule(ctx, b, 2, n, &ms2);
+    double ms[3] = {0, 0, 0};
+    const int n = 8;
+    for (size_t q = 0; q < cand.size() && !rc; ++q) {
+      if (q > 0) rc = restore_snapshot(ctx, b, sn, false);
+      if (!rc) rc = time_schedule(ctx, b, cand[q], n, &ms[cand[q]]);
+    }
     const int rc2 = restore_snapshot(ctx, b, sn, true);
     if (rc || rc2) {
       rc = rc ? rc : rc2;
       break;
     }
-    b.schedule = (ms2 < ms1) ? 2 : 1;
+    int best = cand[0];
+    for (int sc : cand)
+      if (ms[sc] < ms[best]) best = sc;
+    b.schedule = best;
     if (cache_path) {
       std::ofstream out(cache_path, std::ios::app);
-      out << "{\"key\": \"" << key << "\", \"schedule\": " << b.schedule << ", \"ms_two_pass\": " << ms1
-          << ", \"ms_fused\": " << ms2 << "}\n";
+      out << "{\"key\": \"" << key << "\", \"schedule\": " << b.schedule
+          << ", \"ms_two_pass\": " << ms[1] << ", \"ms_fused\": " << ms[2] << "}\n";
     }
   }
   ctx->profiling = prof;

@@ -32,6 +32,7 @@ struct Bucket {
   ChainParams *d_prm = nullptr;
   ChainState *d_st = nullptr;
   void *d_tw = nullptr;
+  double *d_p2 = nullptr;  // -p_x(k)^2 / X
   double *d_part = nullptr, *d_stage = nullptr, *d_spart = nullptr, *d_F = nullptr,
          *d_acc = nullptr;
   int64_t *d_count = nullptr;

@@ -20,6 +20,7 @@ struct BucketArgs {
   ChainState *st;               // controller state read by this launch
   ChainState *st_next;          // written by the fused kernel (== st otherwise)
   const C2<T> *tw;              // tw[m] = exp(-2 pi i m / X)
+  const double *p2;             // p2[k] = -p_x(k)^2 / X (FP64, from the host)
   double *part_rd;              // [chain][Nt][ntiles][2] (max, sum) of the current fields
   double *part_wr;              // partials written by this launch
   double *obs_stage;            // [chain][nchunks][Vs][3]
@@ -43,12 +44,6 @@ __device__ __forceinline__ bool evaluates(const BucketArgs<T> &A, const ChainPar
   const int ev = A.eval_every < 0 ? P.eval_every : A.eval_every;
   if (A.eval_every < 0 && (ev <= 0 || st.cur_step < P.eval_from)) return false;
   return ev <= 0 || (st.cur_step % ev) == 0;
-}
-
-// p of compute_kinetic_part (Propagator_without_cutoff.cu:169-176).
-__device__ __forceinline__ double mom(int j, int n) {
-  if (j < n / 2 || n == 1) return 6.283185307179586 * double(j) / double(n);
-  return -6.283185307179586 * double(n - j) / double(n);
 }
 
 template <typename T>
@@ -88,90 +83,108 @@ __device__ __forceinline__ void store4(C2<T> *f, int64_t site, const C2<T> v[4])
   }
 }
 
-// Per-chain coefficients in the field precision.
+// Per-chain coefficients of the drift in the field precision, with
+// a_x * factor * eps folded in (the kinetic multiplier table carries the same
+// factor, so K' below is a_x factor eps K).
 template <typename T>
 struct Coef {
-  T s[4];   // mu+P+Q, mu, mu, mu-P+Q
-  T G2, c1, eps, af;
+  T af;       // a_x factor
+  T cs[4];    // a_x factor eps s_i,  s = (mu+P+Q, mu, mu, mu-P+Q)
+  T kA, kB;   // a_x factor eps (2G + c1),  a_x factor eps (2G + 2 c1)
   int q3;
 };
 
 template <typename T>
 __device__ __forceinline__ Coef<T> coef(const ChainParams &p) {
   Coef<T> c;
-  c.s[0] = T(p.mu + p.P + p.Q);
-  c.s[1] = T(p.mu);
-  c.s[2] = T(p.mu);
-  c.s[3] = T(p.mu - p.P + p.Q);
-  c.G2 = T(2.0 * p.G);
-  c.c1 = T(p.c1);
-  c.eps = T(p.eps);
+  const double e = p.a_x * p.factor * p.eps;
   c.af = T(p.a_x * p.factor);
+  c.cs[0] = T(e * (p.mu + p.P + p.Q));
+  c.cs[1] = T(e * p.mu);
+  c.cs[2] = T(e * p.mu);
+  c.cs[3] = T(e * (p.mu - p.P + p.Q));
+  c.kA = T(e * (2.0 * p.G + p.c1));
+  c.kB = T(e * (2.0 * p.G + 2.0 * p.c1));
   c.q3 = p.q3_mode;
   return c;
 }
 
-// Three-term sums of N_j without the opposite component (tempm_i / tempp_i,
-// Propagator_without_cutoff.cu:35-57), in the reference's order.
+// Multiplier-table scale of the drift kernels: a_x factor eps / a_x.
+__device__ __forceinline__ double drift_table_scale(const ChainParams &p) { return p.factor * p.eps; }
+
+// sum_j a_j b_j
 template <typename T>
-__device__ __forceinline__ C2<T> nsum3(int i, const C2<T> N[4]) {
-  switch (i) {
-    case 0: return cadd(cadd(N[0], N[1]), N[2]);
-    case 1: return cadd(cadd(N[0], N[1]), N[3]);
-    case 2: return cadd(cadd(N[0], N[2]), N[3]);
-    default: return cadd(cadd(N[1], N[2]), N[3]);
-  }
+__device__ __forceinline__ C2<T> cdot4(const C2<T> a[4], const C2<T> b[4]) {
+  return {a[0].x * b[0].x - a[0].y * b[0].y + a[1].x * b[1].x - a[1].y * b[1].y +
+              a[2].x * b[2].x - a[2].y * b[2].y + a[3].x * b[3].x - a[3].y * b[3].y,
+          a[0].x * b[0].y + a[0].y * b[0].x + a[1].x * b[1].y + a[1].y * b[1].x +
+              a[2].x * b[2].y + a[2].y * b[2].x + a[3].x * b[3].y + a[3].y * b[3].x};
+}
+
+// det [[a0, a1], [a2, a3]] = a0 a3 - a1 a2
+template <typename T>
+__device__ __forceinline__ C2<T> cdet(const C2<T> a[4]) {
+  return {a[0].x * a[3].x - a[0].y * a[3].y - (a[1].x * a[2].x - a[1].y * a[2].y),
+          a[0].x * a[3].y + a[0].y * a[3].x - (a[1].x * a[2].y + a[1].y * a[2].x)};
+}
+
+// One component of a drift: af (f1 - f0) + kin + c f1' + B cof, with
+// f1' = f1 except in the psibar equation of the reference Q3 mode.
+template <typename T>
+__device__ __forceinline__ C2<T> drift_term(T af, C2<T> f1, C2<T> f0, C2<T> kin, C2<T> c, C2<T> f1c,
+                                            C2<T> B, C2<T> cof) {
+  return {af * (f1.x - f0.x) + kin.x + (c.x * f1c.x - c.y * f1c.y) + (B.x * cof.x - B.y * cof.y),
+          af * (f1.y - f0.y) + kin.y + (c.x * f1c.y + c.y * f1c.x) + (B.x * cof.y + B.y * cof.x)};
 }
 
 // Drift of psi and psibar at one site, all four components
-// (calculate_drift_without_noise, Propagator_without_cutoff.cu:1-129) plus
-// the spin term -c1 S_i and the Q3 switch.
+// (calculate_drift_without_noise, Propagator_without_cutoff.cu:1-129, with
+// factor = f) plus the spin term -c1 S_i and the Q3 switch:
+//   D_i  = a_x f [psi_i(t-1) - psi_i(t) + eps (s_i psi_i(t-1) - 2G T-_i - c1 S-_i + K_i(t-1))]
+//   Db_i = a_x f [psib_i(t+1) - psib_i(t) + eps (s_i x_i - 2G T+_i - c1 S+_i + Kb_i(t+1))]
+// with x = psi(t+1) (reference, Q3) or psibar(t+1).  The interaction is
+// evaluated through the invariants of the 2x2 matrices Psi = [[psi_0, psi_1],
+// [psi_2, psi_3]] and Psibar on the time-split pair (Psi(t-1), Psibar(t)):
+// with n_j = psi_j psib_j, N = sum_j n_j, det = det Psi and the cofactors
+// cof = (psib_3, -psib_2, -psib_1, psib_0) = d(det Psibar)/d psib_i,
+//   T_i = (sum_{j != opp(i)} n_j) psi_i + psi_a psib_b psi_c = N psi_i - det cof_i
+//   S_i = F_z (sigma_z psi)_i + F_-+ (sigma_+- psi)_i    = N psi_i - 2 det cof_i
+// (the second from the Fierz identity sum_a F_a^2 = N^2 - 4 det Psi det Psibar),
+// so -2G T_i - c1 S_i = -(2G + c1) N psi_i + (2G + 2 c1) det cof_i.  The
+// psibar equation is the same with the roles of Psi(t) and Psibar(t+1) swapped.
 //   pm = psi(t-1), p0 = psi(t), pp = psi(t+1), b0 = psibar(t), bp = psibar(t+1)
-//   km = K(t-1), kp = Kbar(t+1)
+//   km = K'(t-1), kp = Kb'(t+1) (already scaled by a_x f eps)
 template <typename T>
 __device__ __forceinline__ void drift_site(const Coef<T> &k, const C2<T> pm[4], const C2<T> p0[4],
                                            const C2<T> pp[4], const C2<T> b0[4], const C2<T> bp[4],
                                            const C2<T> km[4], const C2<T> kp[4], C2<T> D[4],
                                            C2<T> Db[4]) {
-  C2<T> Nm[4], Np[4];
+  {
+    const C2<T> N = cdot4(pm, b0), det = cdet(pm);
+    const C2<T> B = {k.kB * det.x, k.kB * det.y};
+    const T ay = -(k.kA * N.y), ax = k.kA * N.x;
+    const C2<T> cof[4] = {b0[3], {-b0[2].x, -b0[2].y}, {-b0[1].x, -b0[1].y}, b0[0]};
 #pragma unroll
-  for (int c = 0; c < 4; ++c) {
-    Nm[c] = cmul(pm[c], b0[c]);
-    Np[c] = cmul(p0[c], bp[c]);
+    for (int i = 0; i < 4; ++i)
+      D[i] = drift_term(k.af, pm[i], p0[i], km[i], C2<T>{k.cs[i] - ax, ay}, pm[i], B, cof[i]);
   }
-  // triples (a,b,c)_i: 0:(1,3,2) 1:(0,2,3) 2:(3,1,0) 3:(2,0,1)
-  constexpr int ta[4] = {1, 0, 3, 2}, tb[4] = {3, 2, 1, 0}, tcx[4] = {2, 3, 0, 1};
-  C2<T> Sm[4] = {}, Sp[4] = {};
-  if (k.c1 != T(0)) {
-    // F+- = Fx +- i Fy, Fz of (psibar(t), psi(t-1)) and (psibar(t+1), psi(t))
-    const C2<T> Fpm = cscale(T(2), cadd(cmul(b0[0], pm[2]), cmul(b0[1], pm[3])));
-    const C2<T> Fmm = cscale(T(2), cadd(cmul(b0[2], pm[0]), cmul(b0[3], pm[1])));
-    const C2<T> Fzm = csub(cadd(Nm[0], Nm[1]), cadd(Nm[2], Nm[3]));
-    Sm[0] = cadd(cmul(Fmm, pm[2]), cmul(Fzm, pm[0]));
-    Sm[1] = cadd(cmul(Fmm, pm[3]), cmul(Fzm, pm[1]));
-    Sm[2] = csub(cmul(Fpm, pm[0]), cmul(Fzm, pm[2]));
-    Sm[3] = csub(cmul(Fpm, pm[1]), cmul(Fzm, pm[3]));
-    const C2<T> Fpp = cscale(T(2), cadd(cmul(bp[0], p0[2]), cmul(bp[1], p0[3])));
-    const C2<T> Fmp = cscale(T(2), cadd(cmul(bp[2], p0[0]), cmul(bp[3], p0[1])));
-    const C2<T> Fzp = csub(cadd(Np[0], Np[1]), cadd(Np[2], Np[3]));
-    Sp[0] = cadd(cmul(Fpp, bp[2]), cmul(Fzp, bp[0]));
-    Sp[1] = cadd(cmul(Fpp, bp[3]), cmul(Fzp, bp[1]));
-    Sp[2] = csub(cmul(Fmp, bp[0]), cmul(Fzp, bp[2]));
-    Sp[3] = csub(cmul(Fmp, bp[1]), cmul(Fzp, bp[3]));
-  }
+  {
+    const C2<T> N = cdot4(p0, bp), det = cdet(bp);
+    const C2<T> B = {k.kB * det.x, k.kB * det.y};
+    const T ax = k.kA * N.x, ay = k.kA * N.y;
+    const C2<T> cof[4] = {p0[3], {-p0[2].x, -p0[2].y}, {-p0[1].x, -p0[1].y}, p0[0]};
+    if (k.q3 == 1) {
 #pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    const C2<T> tm = cadd(cmul(nsum3(i, Nm), pm[i]), cmul(cmul(pm[ta[i]], b0[tb[i]]), pm[tcx[i]]));
-    const C2<T> tp = cadd(cmul(nsum3(i, Np), bp[i]), cmul(cmul(bp[ta[i]], p0[tb[i]]), bp[tcx[i]]));
-    // psi drift
-    C2<T> inter = csub(cscale(-k.G2, tm), cscale(k.c1, Sm[i]));
-    C2<T> inner = cadd(cadd(cscale(k.s[i], pm[i]), inter), km[i]);
-    D[i] = cscale(k.af, cadd(csub(pm[i], p0[i]), cscale(k.eps, inner)));
-    // psibar drift; Q3: psi(t+1) multiplies s_i in the reference mode
-    const C2<T> xq = (k.q3 == 1) ? bp[i] : pp[i];
-    inter = csub(cscale(-k.G2, tp), cscale(k.c1, Sp[i]));
-    inner = cadd(cadd(cscale(k.s[i], xq), inter), kp[i]);
-    Db[i] = cscale(k.af, cadd(csub(bp[i], b0[i]), cscale(k.eps, inner)));
+      for (int i = 0; i < 4; ++i)
+        Db[i] = drift_term(k.af, bp[i], b0[i], kp[i], C2<T>{k.cs[i] - ax, -ay}, bp[i], B, cof[i]);
+    } else {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        // s_i psi(t+1) - A psibar(t+1)
+        const C2<T> kin = {kp[i].x + k.cs[i] * pp[i].x, kp[i].y + k.cs[i] * pp[i].y};
+        Db[i] = drift_term(k.af, bp[i], b0[i], kin, C2<T>{-ax, -ay}, bp[i], B, cof[i]);
+      }
+    }
   }
 }
 
@@ -179,19 +192,10 @@ __device__ __forceinline__ void drift_site(const Coef<T> &k, const C2<T> pm[4], 
 template <typename T>
 __device__ __forceinline__ C2<T> drift0_site(const Coef<T> &k, const C2<T> pm[4], C2<T> p00,
                                              const C2<T> b0[4], C2<T> km0) {
-  C2<T> Nm[4];
-#pragma unroll
-  for (int c = 0; c < 4; ++c) Nm[c] = cmul(pm[c], b0[c]);
-  C2<T> S0 = {T(0), T(0)};
-  if (k.c1 != T(0)) {
-    const C2<T> Fmm = cscale(T(2), cadd(cmul(b0[2], pm[0]), cmul(b0[3], pm[1])));
-    const C2<T> Fzm = csub(cadd(Nm[0], Nm[1]), cadd(Nm[2], Nm[3]));
-    S0 = cadd(cmul(Fmm, pm[2]), cmul(Fzm, pm[0]));
-  }
-  const C2<T> tm = cadd(cmul(nsum3(0, Nm), pm[0]), cmul(cmul(pm[1], b0[3]), pm[2]));
-  const C2<T> inter = csub(cscale(-k.G2, tm), cscale(k.c1, S0));
-  const C2<T> inner = cadd(cadd(cscale(k.s[0], pm[0]), inter), km0);
-  return cscale(k.af, cadd(csub(pm[0], p00), cscale(k.eps, inner)));
+  const C2<T> N = cdot4(pm, b0), det = cdet(pm);
+  const C2<T> B = {k.kB * det.x, k.kB * det.y};
+  return drift_term(k.af, pm[0], p00, km0, C2<T>{k.cs[0] - k.kA * N.x, -(k.kA * N.y)}, pm[0], B,
+                    b0[3]);
 }
 
 // Shared-memory carve-up of the row-transform kernels: twiddles, the
@@ -219,14 +223,15 @@ struct Smem {
   }
 };
 
+// Twiddles and the kinetic multiplier m(k) = scale * (-p_x(k)^2 / X) into LDS:
+// scale = 1/a_x gives K (compute_kinetic_part + normalize, Propagator_without_cutoff.cu:160-203,
+// 423-424); the drift kernels use drift_table_scale.
 template <typename T, int X>
-__device__ __forceinline__ void load_tables(Smem<T, X> &sm, const C2<T> *tw, double a_x, int tid,
-                                            int nthr) {
-  const double norm = 1.0 / (a_x * double(X));
+__device__ __forceinline__ void load_tables(Smem<T, X> &sm, const C2<T> *tw, const double *p2,
+                                            double scale, int tid, int nthr) {
   for (int e = tid; e < X; e += nthr) {
     sm.tw[e] = tw[e];
-    const double p = mom(e, X);
-    sm.mult[e] = T(-(p * p) * norm);
+    sm.mult[e] = T(p2[e] * scale);
   }
 }
 
