@@ -275,7 +275,9 @@ struct RowFFT {
   static constexpr int PAD = row_pad(X, N2);  // LDS cells per line
   __device__ static __forceinline__ int pos(int p) { return p + p / N2; }
 
-  // lines l < nl at buf + l*PAD; tw[m] = exp(-2 pi i m / X); mult[k] = m(k).
+  // lines l < nl at buf + l*PAD; mult[k] = m(k); twiddles in the order each
+  // level reads them, so the lanes of one access hit consecutive cells:
+  //   tw[k1*N2 + n2] = exp(-2 pi i n2 k1 / X),  tw[X + m2*N1 + k1] = exp(+2 pi i m2 k1 / X).
   // Caller syncs before (data written) and reads after the trailing sync.
   __device__ static __forceinline__ void kinetic(C2<T> *buf, const C2<T> *tw, const T *mult,
                                                  int nl, int tid, int nthr) {
@@ -287,7 +289,7 @@ struct RowFFT {
       for (int n1 = 0; n1 < N1; ++n1) v[n1] = L[pos(n2 + N2 * n1)];
       dft<N1, -1>(v);
 #pragma unroll
-      for (int k1 = 1; k1 < N1; ++k1) v[k1] = cmul(v[k1], tw[(n2 * k1) % X]);
+      for (int k1 = 1; k1 < N1; ++k1) v[k1] = cmul(v[k1], tw[k1 * N2 + n2]);
 #pragma unroll
       for (int k1 = 0; k1 < N1; ++k1) L[pos(n2 + N2 * k1)] = v[k1];
     }
@@ -303,7 +305,7 @@ struct RowFFT {
       for (int k2 = 0; k2 < N2; ++k2) v[k2] = cscale(mult[k1 + N1 * k2], v[k2]);
       dft<N2, +1>(v);
 #pragma unroll
-      for (int m2 = 1; m2 < N2; ++m2) v[m2] = cmul(v[m2], cconj(tw[(m2 * k1) % X]));
+      for (int m2 = 1; m2 < N2; ++m2) v[m2] = cmul(v[m2], tw[X + m2 * N1 + k1]);
 #pragma unroll
       for (int m2 = 0; m2 < N2; ++m2) L[pos(m2 + N2 * k1)] = v[m2];
     }

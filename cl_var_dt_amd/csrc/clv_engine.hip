@@ -149,7 +149,7 @@ __global__ __launch_bounds__(64) void k_controller(BucketArgs<T> A, int mode, do
 // + fields += drift (Propagator_without_cutoff.cu:131-157, 432-441) with the
 // drift recomputed in registers and the noise drawn from Philox in registers.
 template <typename T, int X>
-__global__ __launch_bounds__(256) void k_update(BucketArgs<T> A) {
+__global__ __launch_bounds__(256, sizeof(T) == 4 ? 5 : 2) void k_update(BucketArgs<T> A) {
   extern __shared__ __align__(16) char smem_raw[];
   const int chain = A.chain0 + blockIdx.z;
   const ChainParams &P = A.prm[chain];
@@ -221,15 +221,16 @@ __global__ __launch_bounds__(256) void k_update(BucketArgs<T> A) {
 }
 
 // Fused single-pass step (schedule 2).  One workgroup owns `rows` rows of one
-// chain for all Nt slices and marches along tau, updating in place:
+// chain for all Nt slices and marches along tau:
 //   1. dt of this step from the partials of the current fields (the same
 //      fixed-order reduction as k_controller; workgroup 0 commits the state),
 //   2. per tau: K(t) and Kbar(t+1) of the old fields and K_0 of the new
 //      psi_0(t-1) in one LDS batch of 9 row sets, drift + Philox noise +
 //      update, and |D_0| of the NEW fields at t-1 -> next step's partials.
-// The new tau = 0 slice is held in registers until the old one has been read
-// as the wrap-around neighbour of tau = Nt-1.  Reads and writes the state once
-// per step; results are bit-identical to the two-pass schedule.
+// Old fields are read from the current buffer and the new ones written to the
+// other (as the two-pass update does), so the wrap-around neighbours of
+// tau = Nt-1 are still the old tau = 0.  Reads and writes the state once per
+// step; results are bit-identical to the two-pass schedule.
 template <typename T, int X>
 __global__ __launch_bounds__(256, sizeof(T) == 4 ? CLV_FUSED_WAVES_PER_EU : 2) void k_fused(BucketArgs<T> A) {
   extern __shared__ __align__(16) char smem_raw[];
@@ -267,6 +268,7 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? CLV_FUSED_WAVES_PER_EU : 2) v
         if (clamp < 0) ns.clamp_low += 1;
         if (clamp > 0) ns.clamp_high += 1;
         ns.in_parity = st.parity;
+        ns.parity = 1 - st.parity;
         A.st_next[chain] = ns;
       }
     }
@@ -278,14 +280,13 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? CLV_FUSED_WAVES_PER_EU : 2) v
   const bool has_site = j < rows;
   const int r = tile * rows + j;
   const int par = st.parity;
-  C2<T> *psi = (par ? A.buf1_psi : A.buf0_psi) + chain * A.chain_stride;
-  C2<T> *psib = (par ? A.buf1_psib : A.buf0_psib) + chain * A.chain_stride;
+  const C2<T> *psi = (par ? A.buf1_psi : A.buf0_psi) + chain * A.chain_stride;
+  const C2<T> *psib = (par ? A.buf1_psib : A.buf0_psib) + chain * A.chain_stride;
+  C2<T> *opsi = (par ? A.buf0_psi : A.buf1_psi) + chain * A.chain_stride;
+  C2<T> *opsib = (par ? A.buf0_psib : A.buf1_psib) + chain * A.chain_stride;
   const Coef<T> k = coef<T>(P);
   const T dtT = T(dt);
   const T sq = T(sqrt(dt));
-  // the new tau = 0 slice waits in LDS until the old one has been read as
-  // the wrap-around neighbour of tau = Nt-1
-  C2<T> *keep = sm.red_end();
   auto site_at = [&](int t) { return (int64_t(t) * A.R + r) * X + x; };
   // Registers carried along tau (old = fields of this step, new = updated):
   //   pm = old psi(t-1), km = K(t-1), nprev = new psi(t-1),
@@ -316,7 +317,7 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? CLV_FUSED_WAVES_PER_EU : 2) v
     C2<T> pp[4], bn[4];
     if (has_site) {
       load4(psi, site_at(tp1), pp);
-      // old psibar(t+2); at t = Nt-2 that is tau = 0, still unwritten (kept in LDS)
+      // old psibar(t+2) (tau = 0 at t = Nt-2)
       if (t + 2 <= Nt) load4(psib, site_at(t + 2 == Nt ? 0 : t + 2), bn);
 #pragma unroll
       for (int c = 0; c < 4; ++c) {
@@ -353,16 +354,10 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? CLV_FUSED_WAVES_PER_EU : 2) v
         o[c] = cadd(p0[c], cadd(cscale(dtT, D[c]), eta));
         ob[c] = cadd(b0[c], cadd(cscale(dtT, Db[c]), cconj(eta)));
       }
-      if (t == 0) {
-#pragma unroll
-        for (int c = 0; c < 4; ++c) {
-          keep[(c * rows + j) * X + x] = o[c];
-          keep[((4 + c) * rows + j) * X + x] = ob[c];
-        }
-      } else {
-        const int64_t s0 = site_at(t);
-        store4(psi, s0, o);
-        store4(psib, s0, ob);
+      const int64_t s0 = site_at(t);
+      store4(opsi, s0, o);
+      store4(opsib, s0, ob);
+      if (t > 0) {
         const C2<T> d0 = drift0_site(k, nprev, o[0], ob, k0n);
         const double dx = double(d0.x), dy = double(d0.y);
         v = sqrt(dx * dx + dy * dy);
@@ -389,17 +384,12 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? CLV_FUSED_WAVES_PER_EU : 2) v
       __syncthreads();  // LDS lines are rewritten by the next tau
     }
   }
-  // tau = 0 of the new fields, and its |D_0| (needs the new psi(Nt-1))
+  // |D_0| of the new tau = 0 (needs the new psi(Nt-1)); this thread wrote
+  // the new tau = 0 values of its site at t = 0
   C2<T> n0k[4], nb0k[4];
   if (has_site) {
-#pragma unroll
-    for (int c = 0; c < 4; ++c) {
-      n0k[c] = keep[(c * rows + j) * X + x];
-      nb0k[c] = keep[((4 + c) * rows + j) * X + x];
-    }
-    const int64_t s0 = site_at(0);
-    store4(psi, s0, n0k);
-    store4(psib, s0, nb0k);
+    load4(opsi, site_at(0), n0k);
+    load4(opsib, site_at(0), nb0k);
     sm.at(j, x) = nprev[0];
   }
   __syncthreads();
@@ -638,7 +628,7 @@ void launch_row_kernels(const Bucket &b, hipStream_t s, int which, int chain0, i
     hipLaunchKernelGGL((k_obs<T, X>), dim3(b.ntiles, b.nchunks, nch), block, sh, s, A);
   } else {
     BucketArgs<T> A = make_args<T>(b, chain0, eval_every, true);
-    const size_t sh = Smem<T, X>::bytes(9 * b.rows) + sizeof(C2<T>) * 8 * b.rows * X;
+    const size_t sh = Smem<T, X>::bytes(9 * b.rows);
     hipLaunchKernelGGL((k_fused<T, X>), dim3(b.ntiles, nch), block, sh, s, A);
   }
 }
@@ -868,7 +858,7 @@ int setup_bucket(clv_ctx *ctx, Bucket &b) {
   }
   if ((rc = dalloc(ctx, b, reinterpret_cast<void **>(&b.d_prm), sizeof(ChainParams) * b.nch))) return rc;
   if ((rc = dalloc(ctx, b, reinterpret_cast<void **>(&b.d_st), 2 * sizeof(ChainState) * b.nch))) return rc;
-  if ((rc = dalloc(ctx, b, &b.d_tw, b.esz * b.X))) return rc;
+  if ((rc = dalloc(ctx, b, &b.d_tw, 2 * b.esz * b.X))) return rc;
   if ((rc = dalloc(ctx, b, reinterpret_cast<void **>(&b.d_p2), sizeof(double) * b.X))) return rc;
   b.part_slot = size_t(2) * b.nch * b.Nt * b.ntiles;
   if ((rc = dalloc(ctx, b, reinterpret_cast<void **>(&b.d_part), sizeof(double) * 2 * b.part_slot)))
@@ -882,13 +872,18 @@ int setup_bucket(clv_ctx *ctx, Bucket &b) {
   if ((rc = dalloc(ctx, b, reinterpret_cast<void **>(&b.d_F), sizeof(double) * 3 * b.nch * b.Vs))) return rc;
   if ((rc = dalloc(ctx, b, reinterpret_cast<void **>(&b.d_acc), sizeof(double) * 5 * b.nch))) return rc;
   if ((rc = dalloc(ctx, b, reinterpret_cast<void **>(&b.d_count), sizeof(int64_t) * b.nch))) return rc;
-  // twiddles exp(-2 pi i m / X), computed in FP64 on the host
-  std::vector<double> tw(2 * b.X);
-  for (int m = 0; m < b.X; ++m) {
-    const double a = 6.283185307179586 * double(m) / double(b.X);
-    tw[2 * m] = std::cos(a);
-    tw[2 * m + 1] = -std::sin(a);
-  }
+  // row-transform twiddles in FP64 on the host, in the order RowFFT reads them:
+  // [k1*N2 + n2] = exp(-2 pi i n2 k1 / X), [X + m2*N1 + k1] = exp(+2 pi i m2 k1 / X)
+  const int n2s = split_n2(b.X), n1s = b.X / n2s;
+  std::vector<double> tw(4 * b.X);
+  for (int k1 = 0; k1 < n1s; ++k1)
+    for (int n2 = 0; n2 < n2s; ++n2) {
+      const double a = 6.283185307179586 * double((n2 * k1) % b.X) / double(b.X);
+      tw[2 * (k1 * n2s + n2)] = std::cos(a);
+      tw[2 * (k1 * n2s + n2) + 1] = -std::sin(a);
+      tw[2 * (b.X + n2 * n1s + k1)] = std::cos(a);
+      tw[2 * (b.X + n2 * n1s + k1) + 1] = std::sin(a);
+    }
   // -p_x(k)^2 / X with p of compute_kinetic_part (Propagator_without_cutoff.cu:169-176)
   std::vector<double> p2(b.X);
   for (int k = 0; k < b.X; ++k) {
@@ -916,9 +911,8 @@ int setup_bucket(clv_ctx *ctx, Bucket &b) {
   b.host_eval.assign(b.nch, 0);
   b.host_eval_from.assign(b.nch, 0);
   b.host_active.assign(b.nch, 1);
-  const size_t lds = size_t(b.esz) * b.X + size_t(b.esz / 2) * (b.X + (b.X & 1)) +
-                     size_t(b.esz) * 9 * b.rows * row_pad(b.X, split_n2(b.X)) + sizeof(double) * 64 +
-                     size_t(b.esz) * 8 * b.rows * b.X;
+  const size_t lds = size_t(b.esz) * 2 * b.X + size_t(b.esz / 2) * (b.X + (b.X & 1)) +
+                     size_t(b.esz) * 9 * b.rows * row_pad(b.X, split_n2(b.X)) + sizeof(double) * 64;
   b.fused_lds = (lds <= 65536) ? lds : 0;
   b.schedule = 1;
   return CLV_OK;

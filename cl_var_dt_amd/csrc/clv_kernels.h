@@ -19,7 +19,7 @@ struct BucketArgs {
   const ChainParams *prm;
   ChainState *st;               // controller state read by this launch
   ChainState *st_next;          // written by the fused kernel (== st otherwise)
-  const C2<T> *tw;              // tw[m] = exp(-2 pi i m / X)
+  const C2<T> *tw;              // 2X row-transform twiddles (RowFFT order)
   const double *p2;             // p2[k] = -p_x(k)^2 / X (FP64, from the host)
   double *part_rd;              // [chain][Nt][ntiles][2] (max, sum) of the current fields
   double *part_wr;              // partials written by this launch
@@ -209,7 +209,7 @@ struct Smem {
   double *red;
   __device__ Smem(char *base, int nlines) {
     tw = reinterpret_cast<C2<T> *>(base);
-    mult = reinterpret_cast<T *>(tw + X);
+    mult = reinterpret_cast<T *>(tw + 2 * X);
     buf = reinterpret_cast<C2<T> *>(mult + X + (X & 1));
     red = reinterpret_cast<double *>(buf + nlines * F::PAD);
   }
@@ -218,7 +218,7 @@ struct Smem {
   // first byte after the reduction scratch (extra per-kernel LDS)
   __device__ __forceinline__ C2<T> *red_end() { return reinterpret_cast<C2<T> *>(red + 64); }
   static size_t bytes(int nlines) {
-    return sizeof(C2<T>) * X + sizeof(T) * (X + (X & 1)) + sizeof(C2<T>) * nlines * F::PAD +
+    return sizeof(C2<T>) * 2 * X + sizeof(T) * (X + (X & 1)) + sizeof(C2<T>) * nlines * F::PAD +
            sizeof(double) * 64;
   }
 };
@@ -231,6 +231,7 @@ __device__ __forceinline__ void load_tables(Smem<T, X> &sm, const C2<T> *tw, con
                                             double scale, int tid, int nthr) {
   for (int e = tid; e < X; e += nthr) {
     sm.tw[e] = tw[e];
+    sm.tw[X + e] = tw[X + e];
     sm.mult[e] = T(p2[e] * scale);
   }
 }
