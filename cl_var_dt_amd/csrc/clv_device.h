@@ -73,18 +73,20 @@ __device__ __forceinline__ uint4 philox10(uint4 c, uint32_t k0, uint32_t k1) {
   return c;
 }
 
-// Two complex N(0,1) draws for (seed, stream, step, site, pair):
+// Two complex N(0, sd^2) draws for (seed, stream, step, site, pair):
 // counter (site, pair | stream << 8, step_lo, step_hi), uniforms
-// (x + 1/2) 2^-32, Box-Muller (1958) on (u0,u1) and (u2,u3).
+// (x + 1/2) 2^-32, Box-Muller (1958) on (u0,u1) and (u2,u3), the standard
+// deviation sd folded into the radius.
 __device__ __forceinline__ void normals4(double out[4], uint64_t seed, uint32_t stream,
-                                         uint64_t step, uint32_t site, uint32_t pair) {
+                                         uint64_t step, uint32_t site, uint32_t pair,
+                                         double sd = 1.0) {
   const uint4 r = philox10(make_uint4(site, pair | (stream << 8), uint32_t(step), uint32_t(step >> 32)),
                            uint32_t(seed), uint32_t(seed >> 32));
   const double s = 2.3283064365386963e-10;  // 2^-32
   const double u0 = (double(r.x) + 0.5) * s, u1 = (double(r.y) + 0.5) * s;
   const double u2 = (double(r.z) + 0.5) * s, u3 = (double(r.w) + 0.5) * s;
-  const double r0 = sqrt(-2.0 * log(u0));
-  const double r1 = sqrt(-2.0 * log(u2));
+  const double r0 = sd * sqrt(-2.0 * log(u0));
+  const double r1 = sd * sqrt(-2.0 * log(u2));
   double s0, c0, s1, c1;
   sincos(6.283185307179586 * u1, &s0, &c0);
   sincos(6.283185307179586 * u3, &s1, &c1);
@@ -95,7 +97,8 @@ __device__ __forceinline__ void normals4(double out[4], uint64_t seed, uint32_t 
 }
 
 __device__ __forceinline__ void normals4(float out[4], uint64_t seed, uint32_t stream,
-                                         uint64_t step, uint32_t site, uint32_t pair) {
+                                         uint64_t step, uint32_t site, uint32_t pair,
+                                         float sd = 1.0f) {
   const uint4 r = philox10(make_uint4(site, pair | (stream << 8), uint32_t(step), uint32_t(step >> 32)),
                            uint32_t(seed), uint32_t(seed >> 32));
   // FP32 uniforms from the same 32-bit words as the FP64 path; the hardware
@@ -104,8 +107,8 @@ __device__ __forceinline__ void normals4(float out[4], uint64_t seed, uint32_t s
   const float h = 1.16415321826934814e-10f;
   const float u0 = fmaf(float(r.x), s, h), u1 = fmaf(float(r.y), s, h);
   const float u2 = fmaf(float(r.z), s, h), u3 = fmaf(float(r.w), s, h);
-  const float r0 = __builtin_amdgcn_sqrtf(-1.38629436111989061f * __builtin_amdgcn_logf(u0));
-  const float r1 = __builtin_amdgcn_sqrtf(-1.38629436111989061f * __builtin_amdgcn_logf(u2));
+  const float r0 = sd * __builtin_amdgcn_sqrtf(-1.38629436111989061f * __builtin_amdgcn_logf(u0));
+  const float r1 = sd * __builtin_amdgcn_sqrtf(-1.38629436111989061f * __builtin_amdgcn_logf(u2));
   out[0] = r0 * __builtin_amdgcn_cosf(u1);
   out[1] = r0 * __builtin_amdgcn_sinf(u1);
   out[2] = r1 * __builtin_amdgcn_cosf(u3);

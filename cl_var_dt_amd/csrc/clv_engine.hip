@@ -202,11 +202,9 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? 5 : 2) void k_update(BucketAr
   T nz[8] = {};
   if (P.noise_on) {
     const uint32_t site = uint32_t((int64_t(t) * X + x) * A.R + r);  // reference site order
-    normals4(nz, P.seed, 0u, step, site, 0u);
-    normals4(nz + 4, P.seed, 0u, step, site, 1u);
     const T sq = T(sqrt(dt));
-#pragma unroll
-    for (int q = 0; q < 8; ++q) nz[q] *= sq;
+    normals4(nz, P.seed, 0u, step, site, 0u, sq);
+    normals4(nz + 4, P.seed, 0u, step, site, 1u, sq);
   }
   const T dtT = T(dt);
   C2<T> o[4], ob[4];
@@ -342,10 +340,8 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? CLV_FUSED_WAVES_PER_EU : 2) v
       T nz[8] = {};
       if (P.noise_on) {
         const uint32_t site = uint32_t((int64_t(t) * X + x) * A.R + r);
-        normals4(nz, P.seed, 0u, step, site, 0u);
-        normals4(nz + 4, P.seed, 0u, step, site, 1u);
-#pragma unroll
-        for (int q = 0; q < 8; ++q) nz[q] *= sq;
+        normals4(nz, P.seed, 0u, step, site, 0u, sq);
+        normals4(nz + 4, P.seed, 0u, step, site, 1u, sq);
       }
       C2<T> o[4], ob[4];
 #pragma unroll
