@@ -161,6 +161,41 @@ def test_drift_is_minus_action_gradient(oracle_lib, c1, dims):
         assert abs(Db[idx] + dS_dq) < 1e-10 * max(1, abs(dS_dq)), idx
 
 
+def test_drift_invariant_form():
+    """The identities the GPU kernels evaluate the interaction with, on random
+    complex values (no oracle involved):
+      T_i = (sum_{j != opp(i)} n_j) psi_i + psi_a psib_b psi_c     (Propagator_without_cutoff.cu:35-57)
+          = N psi_i - det(Psi) cof_i(Psibar)
+      S_i = sum_a F_a d F_a / d psib_i  (F_a = sum_col psib_{r,col} sigma^a_{r r'} psi_{r',col})
+          = N psi_i - 2 det(Psi) cof_i(Psibar)
+    with N = sum_j psi_j psib_j, Psi = [[psi_0, psi_1], [psi_2, psi_3]] (row = spin),
+    cof = (psib_3, -psib_2, -psib_1, psib_0), and sum_a F_a^2 = N^2 - 4 det Psi det Psibar."""
+    rng = np.random.default_rng(11)
+    sig = [np.array([[0, 1], [1, 0]], complex), np.array([[0, -1j], [1j, 0]]),
+           np.array([[1, 0], [0, -1]], complex)]
+    opp = [3, 2, 1, 0]
+    trip = [(1, 3, 2), (0, 2, 3), (3, 1, 0), (2, 0, 1)]
+    for _ in range(50):
+        q = rng.normal(size=4) + 1j * rng.normal(size=4)   # psi (independent of psibar)
+        b = rng.normal(size=4) + 1j * rng.normal(size=4)   # psibar
+        n = q * b
+        N = n.sum()
+        det_q = q[0] * q[3] - q[1] * q[2]
+        det_b = b[0] * b[3] - b[1] * b[2]
+        cof = np.array([b[3], -b[2], -b[1], b[0]])
+        Q = q.reshape(2, 2)
+        B = b.reshape(2, 2)
+        F = [np.einsum("rc,rs,sc->", B, s_, Q) for s_ in sig]
+        assert abs(sum(f * f for f in F) - (N * N - 4 * det_q * det_b)) < 1e-12 * abs(N * N) + 1e-12
+        for i in range(4):
+            a, bb, c = trip[i]
+            T_ref = sum(n[j] for j in range(4) if j != opp[i]) * q[i] + q[a] * b[bb] * q[c]
+            assert abs(T_ref - (N * q[i] - det_q * cof[i])) < 1e-12 * (1 + abs(T_ref))
+            r, col = divmod(i, 2)
+            S_def = sum(F[k] * (sig[k][r, :] @ Q[:, col]) for k in range(3))
+            assert abs(S_def - (N * q[i] - 2 * det_q * cof[i])) < 1e-12 * (1 + abs(S_def))
+
+
 def test_drift_q3_reference_mode(oracle_lib):
     """Q3: in the reference the psibar drift's s_i term multiplies psi(t+1)."""
     dims = (8, 1, 1, 4)
