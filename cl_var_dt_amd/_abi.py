@@ -27,7 +27,7 @@ SCHED_TWO_PASS = 1
 SCHED_FUSED = 2
 
 KERNEL_KINDS = {"drift_stats": 0, "controller": 1, "update": 2, "obs": 3, "obs_finish": 4,
-                "fused": 5}
+                "fused": 5, "transverse": 6}
 
 
 class Shape(C.Structure):

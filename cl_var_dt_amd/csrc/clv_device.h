@@ -37,7 +37,9 @@ struct ChainParams {
   double mu, P, G, c1, Q, eps, a_x, DT, dt_min, dt_max, alpha, factor;
   uint64_t seed;
   int32_t q3_mode, noise_on, active;
+  int32_t kin_dims;    // 1: p^2 = p_x^2; 3: full Laplacian (transverse term in kt)
   int32_t eval_every;  // per-chain observable cadence (serving); 0 never
+  int32_t pad;
   int64_t eval_from;   // first step index a per-chain cadence may evaluate
 };
 
@@ -263,11 +265,13 @@ template <> struct Split<256> { static constexpr int N1 = 16, N2 = 16; };
 // order.  Each level reads and writes the same LDS cells, so one buffer
 // suffices; a line is padded by one cell per N2 so that both access patterns
 // are free of bank conflicts.
-// Cells per line: at least X + X/N2, and congruent to N2 modulo 32 so that
-// the lanes of a first-level task (consecutive n2, consecutive lines) fall in
-// distinct 8-byte LDS bank pairs.
+// Cells per line: at least X + X/N2; for X >= 32 also congruent to N2 modulo
+// 32, so that the lanes of a first-level task (consecutive n2, consecutive
+// lines) fall in distinct 8-byte LDS bank pairs.  Short lines keep the
+// minimum: padding them to that congruence would multiply their LDS footprint.
 __host__ __device__ constexpr int row_pad(int X, int N2) {
   int p = X + X / N2;
+  if (X < 32) return p;
   while (p % 32 != N2 % 32) ++p;
   return p;
 }

@@ -84,7 +84,12 @@ __global__ __launch_bounds__(256) void k_drift_stats(BucketArgs<T> A) {
       C2<T> b0[4];
       load4(psib, (int64_t(t) * A.R + r) * X + x, b0);
       const C2<T> p00 = (q + 1 < TC) ? pm[q + 1][0] : load1(psi, (int64_t(t) * A.R + r) * X + x, 0);
-      const C2<T> d = drift0_site(k, pm[q], p00, b0, sm.at(q * rows + j, x));
+      C2<T> km0 = sm.at(q * rows + j, x);
+      if (has_kt(A, P)) {
+        const int tm1 = (t == 0) ? A.Nt - 1 : t - 1;
+        km0 = add_kt(km0, k.kK, load1(A.kt_psi + chain * A.chain_stride, (int64_t(tm1) * A.R + r) * X + x, 0));
+      }
+      const C2<T> d = drift0_site(k, pm[q], p00, b0, km0);
       const double dx = double(d.x), dy = double(d.y);
       v = sqrt(dx * dx + dy * dy);
     }
@@ -197,6 +202,16 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? 5 : 2) void k_update(BucketAr
   load4(psi, sp1, pp);
   load4(psib, s0, b0);
   const Coef<T> k = coef<T>(P);
+  if (has_kt(A, P)) {
+    C2<T> ktm[4], ktp[4];
+    load4(A.kt_psi + chain * A.chain_stride, (int64_t(tm1) * A.R + r) * X + x, ktm);
+    load4(A.kt_psib + chain * A.chain_stride, sp1, ktp);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      km[c] = add_kt(km[c], k.kK, ktm[c]);
+      kp[c] = add_kt(kp[c], k.kK, ktp[c]);
+    }
+  }
   C2<T> D[4], Db[4];
   drift_site(k, pm, p0, pp, b0, bp, km, kp, D, Db);
   T nz[8] = {};
@@ -450,6 +465,11 @@ __global__ __launch_bounds__(256) void k_obs(BucketArgs<T> A) {
         bb[c] = {double(b[c].x), double(b[c].y)};
         qq[c] = {double(q[c].x), double(q[c].y)};
         kk[c] = {double(kc.x), double(kc.y)};
+        if (has_kt(A, P)) {
+          const C2<T> kt = load1(A.kt_psi + chain * A.chain_stride, (int64_t(t) * A.R + r) * X + x, c);
+          kk[c].x += double(kt.x);
+          kk[c].y += double(kt.y);
+        }
       }
       C2<double> h = {0.0, 0.0}, nn = {0.0, 0.0};
 #pragma unroll
@@ -497,6 +517,71 @@ __global__ __launch_bounds__(256) void k_obs(BucketArgs<T> A) {
     double s = 0.0;
     for (int q = 0; q < nw; ++q) s += sm.red[tid * 8 + q];
     A.obs_spart[((int64_t(chain) * A.nchunks + chunk) * A.ntiles + tile) * 5 + tid] = s;
+  }
+}
+
+// Transverse part of the kinetic term for kin_dims = 3, the full Laplacian
+// (the reference keeps p_x^2 only, Propagator_without_cutoff.cu:196).  The
+// spectral second derivative is a sum of 1-D ones,
+//   K = K_x + K_y + K_z,  K_d = (1/a_x) F_d^-1[-p_d^2 F_d psi] / n_d,
+// so the x part stays in the row transform of the drift kernels and this
+// kernel adds K_d along one axis d in {y, z}, for both fields of the current
+// buffer, into kt (unscaled; the drift kernels weight it by a_x f eps).  A
+// workgroup transforms, for one tau and one position on the other transverse
+// axis, the 4 components of W consecutive x-lines along d: 4W lines of L.
+// Site of line position l: ((tau R + base*bmul + l*rstride) X + x0 + xl).
+template <typename T>
+struct TransArgs {
+  const C2<T> *tw;  // RowFFT twiddles of length L
+  const double *p2; // -p_d(k)^2 / L
+  int X, W, bmul, rstride, nbase;
+  int accumulate;   // add to kt (second axis) instead of overwriting it
+  int gate;         // only chains that evaluate observables after this step
+};
+
+template <typename T, int L>
+__global__ __launch_bounds__(256) void k_transverse(BucketArgs<T> A, TransArgs<T> G) {
+  extern __shared__ __align__(16) char smem_raw[];
+  const int chain = A.chain0 + blockIdx.z;
+  const ChainParams &P = A.prm[chain];
+  if (!P.active || !has_kt(A, P)) return;
+  const ChainState &st = A.st[chain];
+  if (G.gate && !evaluates(A, P, st)) return;
+  const int tid = threadIdx.x, nthr = blockDim.x;
+  const int W = G.W, X = G.X, nxb = X / W;
+  const int xb = blockIdx.x % nxb, base = blockIdx.x / nxb, t = blockIdx.y;
+  Smem<T, L> sm(smem_raw, 4 * W);
+  load_tables<T, L>(sm, G.tw, G.p2, 1.0 / P.a_x, tid, nthr);
+  const int par = st.parity;  // current fields
+  const int64_t row0 = int64_t(t) * A.R + int64_t(base) * G.bmul;
+  for (int f = 0; f < 2; ++f) {
+    const C2<T> *src = (f == 0 ? (par ? A.buf1_psi : A.buf0_psi) : (par ? A.buf1_psib : A.buf0_psib)) +
+                       chain * A.chain_stride;
+    C2<T> *dst = (f == 0 ? A.kt_psi : A.kt_psib) + chain * A.chain_stride;
+    for (int e = tid; e < W * L; e += nthr) {
+      const int xl = e % W, l = e / W;
+      C2<T> v[4];
+      load4(src, (row0 + int64_t(l) * G.rstride) * X + xb * W + xl, v);
+#pragma unroll
+      for (int c = 0; c < 4; ++c) sm.at(c * W + xl, l) = v[c];
+    }
+    __syncthreads();
+    RowFFT<T, L>::kinetic(sm.buf, sm.tw, sm.mult, 4 * W, tid, nthr);
+    for (int e = tid; e < W * L; e += nthr) {
+      const int xl = e % W, l = e / W;
+      const int64_t site = (row0 + int64_t(l) * G.rstride) * X + xb * W + xl;
+      C2<T> v[4];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) v[c] = sm.at(c * W + xl, l);
+      if (G.accumulate) {
+        C2<T> o[4];
+        load4(dst, site, o);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) v[c] = cadd(o[c], v[c]);
+      }
+      store4(dst, site, v);
+    }
+    __syncthreads();  // the lines are rewritten by the next field
   }
 }
 
@@ -582,6 +667,8 @@ BucketArgs<T> make_args(const Bucket &b, int chain0, int eval_every, bool fused 
   A.st_next = fused ? b.d_st + (slot ^ 1) * b.nch : A.st;
   A.tw = static_cast<const C2<T> *>(b.d_tw);
   A.p2 = b.d_p2;
+  A.kt_psi = b.kt_on ? static_cast<C2<T> *>(b.kt[0]) : nullptr;
+  A.kt_psib = b.kt_on ? static_cast<C2<T> *>(b.kt[1]) : nullptr;
   A.part_rd = b.d_part + slot * b.part_slot;
   A.part_wr = fused ? b.d_part + (slot ^ 1) * b.part_slot : A.part_rd;
   A.obs_stage = b.d_stage;
@@ -644,6 +731,37 @@ void launch_row_kernels(const Bucket &b, hipStream_t s, int which, int chain0, i
     case 256: CALL(256); break;                \
     default: break;                            \
   }
+
+// Transverse kinetic term of chains [chain0, chain0+nch), one launch per axis.
+template <typename T, int L>
+void launch_transverse_axis(const Bucket &b, const Bucket::KtAxis &a, hipStream_t s, int chain0,
+                            int nch, int accumulate, int gate, int ev) {
+  const BucketArgs<T> A = make_args<T>(b, chain0, ev);
+  TransArgs<T> G;
+  G.tw = static_cast<const C2<T> *>(a.tw);
+  G.p2 = a.p2;
+  G.X = b.X;
+  G.W = a.W;
+  G.bmul = a.bmul;
+  G.rstride = a.rstride;
+  G.nbase = a.nbase;
+  G.accumulate = accumulate;
+  G.gate = gate;
+  const size_t sh = Smem<T, L>::bytes(4 * a.W);
+  hipLaunchKernelGGL((k_transverse<T, L>), dim3((b.X / a.W) * a.nbase, b.Nt, nch), dim3(256), sh, s,
+                     A, G);
+}
+
+template <typename T>
+void dispatch_transverse(const Bucket &b, hipStream_t s, int chain0, int nch, int gate, int ev) {
+  for (size_t i = 0; i < b.kt_axes.size(); ++i) {
+    const Bucket::KtAxis &a = b.kt_axes[i];
+    const int acc = i > 0 ? 1 : 0;
+#define CLV_TR(LL) launch_transverse_axis<T, LL>(b, a, s, chain0, nch, acc, gate, ev)
+    CLV_SWITCH_X(a.L, CLV_TR)
+#undef CLV_TR
+  }
+}
 
 template <typename T>
 void dispatch_row(const Bucket &b, hipStream_t s, int which, int chain0, int nch, int ev) {
@@ -718,6 +836,14 @@ void row_kernel(clv_ctx *c, const Bucket &b, int which, int chain0, int nch, int
   timed(c, kinds[which], [&] {
     if (c->precision == CLV_FP32) dispatch_row<float>(b, c->stream, which, chain0, nch, ev);
     else dispatch_row<double>(b, c->stream, which, chain0, nch, ev);
+  });
+}
+// gate: only chains that evaluate observables after this step (eval cadence ev)
+void transverse(clv_ctx *c, const Bucket &b, int chain0, int nch, int gate, int ev) {
+  if (!b.kt_on) return;
+  timed(c, 6, [&] {
+    if (c->precision == CLV_FP32) dispatch_transverse<float>(b, c->stream, chain0, nch, gate, ev);
+    else dispatch_transverse<double>(b, c->stream, chain0, nch, gate, ev);
   });
 }
 void controller(clv_ctx *c, const Bucket &b, int chain0, int nch, int mode, double dt, int cl) {
@@ -822,9 +948,70 @@ ChainParams to_device_params(const clv_params &p, int eval_every, int active, in
   d.mu = p.mu; d.P = p.P; d.G = p.G; d.c1 = p.c1; d.Q = p.Q; d.eps = p.eps; d.a_x = p.a_x;
   d.DT = p.DT; d.dt_min = p.dt_min; d.dt_max = p.dt_max; d.alpha = p.alpha; d.factor = p.factor;
   d.seed = p.seed; d.q3_mode = p.q3_mode; d.noise_on = p.noise_on; d.active = active;
+  d.kin_dims = p.kin_dims;
   d.eval_every = eval_every;
+  d.pad = 0;
   d.eval_from = eval_from;
   return d;
+}
+
+// Row-transform tables of lines of length L, computed in FP64 on the host:
+// twiddles in the order RowFFT reads them ([k1*N2 + n2] = exp(-2 pi i n2 k1 / L),
+// [L + m2*N1 + k1] = exp(+2 pi i m2 k1 / L)) in the field precision, and
+// -p(k)^2 / L with p of compute_kinetic_part (Propagator_without_cutoff.cu:169-176).
+int upload_row_tables(clv_ctx *ctx, int L, void *d_tw, double *d_p2) {
+  const int n2s = split_n2(L), n1s = L / n2s;
+  std::vector<double> tw(4 * L);
+  for (int k1 = 0; k1 < n1s; ++k1)
+    for (int n2 = 0; n2 < n2s; ++n2) {
+      const double a = 6.283185307179586 * double((n2 * k1) % L) / double(L);
+      tw[2 * (k1 * n2s + n2)] = std::cos(a);
+      tw[2 * (k1 * n2s + n2) + 1] = -std::sin(a);
+      tw[2 * (L + n2 * n1s + k1)] = std::cos(a);
+      tw[2 * (L + n2 * n1s + k1) + 1] = std::sin(a);
+    }
+  std::vector<double> p2(L);
+  for (int k = 0; k < L; ++k) {
+    const double p = (k < L / 2 || L == 1) ? 6.283185307179586 * double(k) / double(L)
+                                           : -6.283185307179586 * double(L - k) / double(L);
+    p2[k] = -(p * p) / double(L);
+  }
+  HIPCHK(ctx, hipMemcpy(d_p2, p2.data(), sizeof(double) * L, hipMemcpyHostToDevice));
+  if (ctx->precision == CLV_FP32) {
+    std::vector<float> twf(tw.begin(), tw.end());
+    HIPCHK(ctx, hipMemcpy(d_tw, twf.data(), sizeof(float) * twf.size(), hipMemcpyHostToDevice));
+  } else {
+    HIPCHK(ctx, hipMemcpy(d_tw, tw.data(), sizeof(double) * tw.size(), hipMemcpyHostToDevice));
+  }
+  return CLV_OK;
+}
+
+// kin_dims = 3 on this bucket: buffers of the transverse kinetic term and the
+// line tables of the y and z axes longer than 1.
+int enable_kt(clv_ctx *ctx, Bucket &b) {
+  if (b.kt_on) return CLV_OK;
+  const int Y = b.shape.ny, Z = b.shape.nz;
+  std::vector<Bucket::KtAxis> axes;
+  if (Y > 1) axes.push_back({Y, 1, Z, Z, 1});         // lines along y: r = z + y*Z
+  if (Z > 1) axes.push_back({Z, Z, 1, Y, 1});         // lines along z: r = y*Z + z
+  if (axes.empty()) return CLV_OK;                     // 1-D: p_y = p_z = 0
+  const size_t field = size_t(b.nch) * size_t(b.S) * 4 * b.esz;
+  int rc;
+  for (int f = 0; f < 2; ++f)
+    if ((rc = dalloc(ctx, b, &b.kt[f], field))) return rc;
+  const int budget = (b.esz == 8) ? 4096 : 2048;       // complex values per workgroup
+  for (Bucket::KtAxis &a : axes) {
+    if ((rc = dalloc(ctx, b, &a.tw, 2 * b.esz * a.L))) return rc;
+    if ((rc = dalloc(ctx, b, reinterpret_cast<void **>(&a.p2), sizeof(double) * a.L))) return rc;
+    if ((rc = upload_row_tables(ctx, a.L, a.tw, a.p2))) return rc;
+    a.W = 1;  // x-lines per workgroup: the largest divisor of X within the budget
+    for (int w = 1; w <= b.X; ++w)
+      if (b.X % w == 0 && 4 * w * a.L <= budget) a.W = w;
+  }
+  b.kt_axes = axes;
+  b.kt_on = true;
+  drop_graphs(ctx);
+  return CLV_OK;
 }
 
 int setup_bucket(clv_ctx *ctx, Bucket &b) {
@@ -837,6 +1024,16 @@ int setup_bucket(clv_ctx *ctx, Bucket &b) {
   b.esz = (ctx->precision == CLV_FP32) ? 8 : 16;
   if (const char *e = std::getenv("CLV_TILE_SITES")) g_tile_sites = std::max(64, std::min(256, std::atoi(e)));
   b.rows = pick_rows(b.X, b.R, std::max(g_tile_sites, b.X));
+  // the update kernel stages 8 lines per row: keep its LDS within 64 KB
+  const auto update_lds = [&](int rows) {
+    return size_t(b.esz) * 2 * b.X + size_t(b.esz / 2) * (b.X + (b.X & 1)) +
+           size_t(b.esz) * 8 * rows * row_pad(b.X, split_n2(b.X)) + sizeof(double) * 64;
+  };
+  while (b.rows > 1 && update_lds(b.rows) > 65536) {
+    int q = b.rows - 1;
+    while (b.R % q != 0) --q;
+    b.rows = q;
+  }
   b.ntiles = b.R / b.rows;
   b.nthr = ((b.X * b.rows + 63) / 64) * 64;
   b.tc = 1;
@@ -868,32 +1065,7 @@ int setup_bucket(clv_ctx *ctx, Bucket &b) {
   if ((rc = dalloc(ctx, b, reinterpret_cast<void **>(&b.d_F), sizeof(double) * 3 * b.nch * b.Vs))) return rc;
   if ((rc = dalloc(ctx, b, reinterpret_cast<void **>(&b.d_acc), sizeof(double) * 5 * b.nch))) return rc;
   if ((rc = dalloc(ctx, b, reinterpret_cast<void **>(&b.d_count), sizeof(int64_t) * b.nch))) return rc;
-  // row-transform twiddles in FP64 on the host, in the order RowFFT reads them:
-  // [k1*N2 + n2] = exp(-2 pi i n2 k1 / X), [X + m2*N1 + k1] = exp(+2 pi i m2 k1 / X)
-  const int n2s = split_n2(b.X), n1s = b.X / n2s;
-  std::vector<double> tw(4 * b.X);
-  for (int k1 = 0; k1 < n1s; ++k1)
-    for (int n2 = 0; n2 < n2s; ++n2) {
-      const double a = 6.283185307179586 * double((n2 * k1) % b.X) / double(b.X);
-      tw[2 * (k1 * n2s + n2)] = std::cos(a);
-      tw[2 * (k1 * n2s + n2) + 1] = -std::sin(a);
-      tw[2 * (b.X + n2 * n1s + k1)] = std::cos(a);
-      tw[2 * (b.X + n2 * n1s + k1) + 1] = std::sin(a);
-    }
-  // -p_x(k)^2 / X with p of compute_kinetic_part (Propagator_without_cutoff.cu:169-176)
-  std::vector<double> p2(b.X);
-  for (int k = 0; k < b.X; ++k) {
-    const double p = (k < b.X / 2 || b.X == 1) ? 6.283185307179586 * double(k) / double(b.X)
-                                                : -6.283185307179586 * double(b.X - k) / double(b.X);
-    p2[k] = -(p * p) / double(b.X);
-  }
-  HIPCHK(ctx, hipMemcpy(b.d_p2, p2.data(), sizeof(double) * b.X, hipMemcpyHostToDevice));
-  if (ctx->precision == CLV_FP32) {
-    std::vector<float> twf(tw.begin(), tw.end());
-    HIPCHK(ctx, hipMemcpy(b.d_tw, twf.data(), sizeof(float) * twf.size(), hipMemcpyHostToDevice));
-  } else {
-    HIPCHK(ctx, hipMemcpy(b.d_tw, tw.data(), sizeof(double) * tw.size(), hipMemcpyHostToDevice));
-  }
+  if ((rc = upload_row_tables(ctx, b.X, b.d_tw, b.d_p2))) return rc;
   HIPCHK(ctx, hipMemset(b.d_st, 0, 2 * sizeof(ChainState) * b.nch));
   HIPCHK(ctx, hipMemset(b.d_prm, 0, sizeof(ChainParams) * b.nch));
   HIPCHK(ctx, hipMemset(b.d_F, 0, sizeof(double) * 3 * b.nch * b.Vs));
@@ -927,12 +1099,13 @@ void count_step(Bucket &b, int64_t n) {
 }
 
 void step_bucket(clv_ctx *ctx, Bucket &b, int eval_every) {
-  if (b.schedule == 2 && b.fused_lds > 0) {
+  if (uses_fused(b)) {
     if (!b.primed) row_kernel(ctx, b, 0, 0, b.nch, 0);  // partials of the current fields
     row_kernel(ctx, b, 3, 0, b.nch, 0);
     b.epoch ^= 1;  // new state and next step's partials live in the other slot
     b.primed = true;
   } else {
+    transverse(ctx, b, 0, b.nch, 0, 0);
     row_kernel(ctx, b, 0, 0, b.nch, 0);
     controller(ctx, b, 0, b.nch, 0, 0.0, 0);
     row_kernel(ctx, b, 1, 0, b.nch, 0);
@@ -941,21 +1114,24 @@ void step_bucket(clv_ctx *ctx, Bucket &b, int eval_every) {
   const bool ev = bucket_needs_eval(b, eval_every);
   count_step(b, 1);
   if (ev) {
+    transverse(ctx, b, 0, b.nch, 1, eval_every);
     row_kernel(ctx, b, 2, 0, b.nch, eval_every);
     obs_finish(ctx, b, 0, b.nch, eval_every);
   }
 }
 
 void enqueue_step_gated(clv_ctx *ctx, Bucket &b, int eval_every) {
-  if (b.schedule == 2 && b.fused_lds > 0) {
+  if (uses_fused(b)) {
     row_kernel(ctx, b, 3, 0, b.nch, 0);
     b.epoch ^= 1;
   } else {
+    transverse(ctx, b, 0, b.nch, 0, 0);
     row_kernel(ctx, b, 0, 0, b.nch, 0);
     controller(ctx, b, 0, b.nch, 0, 0.0, 0);
     row_kernel(ctx, b, 1, 0, b.nch, 0);
   }
   if (eval_every != 0) {
+    transverse(ctx, b, 0, b.nch, 1, eval_every);
     row_kernel(ctx, b, 2, 0, b.nch, eval_every);
     obs_finish(ctx, b, 0, b.nch, eval_every);
   }
@@ -1015,6 +1191,7 @@ int chain_admit(clv_ctx *ctx, Bucket &b, int local, const clv_params &p, int eva
 }
 
 void chain_evaluate(clv_ctx *ctx, Bucket &b, int local) {
+  transverse(ctx, b, local, 1, 0, 0);
   row_kernel(ctx, b, 2, local, 1, 0);
   obs_finish(ctx, b, local, 1, 0);
 }
@@ -1200,18 +1377,25 @@ int clv_set_params(clv_ctx *ctx, int32_t chain, const clv_params *p) {
   int rc = check_chain(ctx, chain);
   if (rc) return rc;
   if (!p) return fail(ctx, CLV_ERR_ARG, "clv_set_params: NULL params");
-  if (p->kin_dims != 1)
+  if (p->kin_dims != 1 && p->kin_dims != 3)
     return fail(ctx, CLV_ERR_ARG,
-                "clv_set_params: kin_dims must be 1 (p^2 = p_x^2, the reference's kinetic term); "
-                "the full Laplacian is not built on the GPU path");
+                "clv_set_params: kin_dims must be 1 (p^2 = p_x^2, the reference's kinetic term) "
+                "or 3 (p_x^2 + p_y^2 + p_z^2)");
   if (p->q3_mode != CLV_Q3_REFERENCE && p->q3_mode != CLV_Q3_PSIBAR)
     return fail(ctx, CLV_ERR_ARG, "clv_set_params: bad q3_mode");
   if (!(p->a_x > 0.0) || !(p->DT > 0.0) || !(p->eps > 0.0))
     return fail(ctx, CLV_ERR_ARG, "clv_set_params: a_x, DT and eps must be positive");
   (void)hipSetDevice(ctx->device);
-  ctx->params[chain] = *p;
   const auto [bi, local] = ctx->where[chain];
   Bucket &b = ctx->buckets[bi];
+  if (p->kin_dims == 3) {
+    if ((b.shape.ny > 1 && !supported_x(b.shape.ny)) || (b.shape.nz > 1 && !supported_x(b.shape.nz)))
+      return fail(ctx, CLV_ERR_ARG,
+                  "clv_set_params: kin_dims = 3 needs Y and Z in {1, 4, 8, 16, 24, 32, 48, 64, 96, "
+                  "128, 192, 256}");
+    if ((rc = enable_kt(ctx, b))) return rc;
+  }
+  ctx->params[chain] = *p;
   ChainParams d = to_device_params(*p, b.host_eval[local], b.host_active[local],
                                    b.host_eval_from[local]);
   HIPCHK(ctx, hipMemcpyAsync(b.d_prm + local, &d, sizeof(d), hipMemcpyHostToDevice, ctx->stream));
@@ -1350,7 +1534,7 @@ int clv_propagate(clv_ctx *ctx, int64_t nsteps, int32_t eval_every) {
   }
   // fused buckets need the partials of their current fields before replay
   for (Bucket &b : ctx->buckets)
-    if (b.schedule == 2 && b.fused_lds > 0 && !b.primed) {
+    if (uses_fused(b) && !b.primed) {
       row_kernel(ctx, b, 0, 0, b.nch, 0);
       b.primed = true;
     }
@@ -1361,7 +1545,7 @@ int clv_propagate(clv_ctx *ctx, int64_t nsteps, int32_t eval_every) {
   for (int64_t k = 0; k < pairs; ++k) HIPCHK(ctx, hipGraphLaunch(exec, ctx->stream));
   for (Bucket &b : ctx->buckets) {
     count_step(b, 2 * pairs);
-    if (!(b.schedule == 2 && b.fused_lds > 0)) b.primed = false;
+    if (!(uses_fused(b))) b.primed = false;
   }
   for (int64_t n = 2 * pairs; n < nsteps; ++n)
     for (Bucket &b : ctx->buckets) step_bucket(ctx, b, eval_every);
@@ -1375,6 +1559,7 @@ int clv_drift_stats(clv_ctx *ctx, int32_t chain, double *max_drift, double *mean
   (void)hipSetDevice(ctx->device);
   const auto [bi, local] = ctx->where[chain];
   Bucket &b = ctx->buckets[bi];
+  transverse(ctx, b, local, 1, 0, 0);
   row_kernel(ctx, b, 0, local, 1, 0);
   controller(ctx, b, local, 1, 1, 0.0, 0);
   HIPCHK(ctx, hipGetLastError());
@@ -1392,6 +1577,7 @@ int clv_apply_update(clv_ctx *ctx, int32_t chain, double dt) {
   (void)hipSetDevice(ctx->device);
   const auto [bi, local] = ctx->where[chain];
   Bucket &b = ctx->buckets[bi];
+  transverse(ctx, b, local, 1, 0, 0);  // of the fields this update reads
   controller(ctx, b, local, 1, 2, dt, 0);
   row_kernel(ctx, b, 1, local, 1, 0);
   b.host_steps[local] += 1;
@@ -1411,6 +1597,7 @@ int clv_propagate_plain(clv_ctx *ctx, int32_t chain, int64_t nsteps, int32_t eva
   HIPCHK(ctx, hipHostMalloc(reinterpret_cast<void **>(&pinned), sizeof(ChainState)));
   for (int64_t n = 0; n < nsteps; ++n) {
     // drift + max/mean on the device, blocking copy to the host
+    transverse(ctx, b, local, 1, 0, 0);
     row_kernel(ctx, b, 0, local, 1, 0);
     controller(ctx, b, local, 1, 1, 0.0, 0);
     hipError_t e = hipMemcpyAsync(pinned, b.d_st + (b.epoch & 1) * b.nch + local, sizeof(ChainState),
@@ -1429,6 +1616,7 @@ int clv_propagate_plain(clv_ctx *ctx, int32_t chain, int64_t nsteps, int32_t eva
     const bool ev = eval_every > 0 && (b.host_steps[local] % eval_every == 0);
     b.host_steps[local] += 1;
     if (ev) {
+      transverse(ctx, b, local, 1, 1, eval_every);
       row_kernel(ctx, b, 2, local, 1, eval_every);
       obs_finish(ctx, b, local, 1, eval_every);
     }
@@ -1488,6 +1676,7 @@ int clv_evaluate(clv_ctx *ctx, int32_t chain) {
   (void)hipSetDevice(ctx->device);
   if (chain < 0) {
     for (Bucket &b : ctx->buckets) {
+      transverse(ctx, b, 0, b.nch, 0, 0);
       row_kernel(ctx, b, 2, 0, b.nch, 0);
       obs_finish(ctx, b, 0, b.nch, 0);
     }
@@ -1496,6 +1685,7 @@ int clv_evaluate(clv_ctx *ctx, int32_t chain) {
     if (rc) return rc;
     const auto [bi, local] = ctx->where[chain];
     Bucket &b = ctx->buckets[bi];
+    transverse(ctx, b, local, 1, 0, 0);
     row_kernel(ctx, b, 2, local, 1, 0);
     obs_finish(ctx, b, local, 1, 0);
   }
@@ -1634,7 +1824,7 @@ int time_schedule(clv_ctx *ctx, Bucket &b, int schedule, int nsteps, double *ms)
 }
 
 bool schedule_fits(const Bucket &b, int schedule) {
-  return schedule == 1 || (schedule == 2 && b.fused_lds > 0);
+  return schedule == 1 || (schedule == 2 && b.fused_lds > 0 && !b.kt_on);
 }
 
 }  // namespace
@@ -1698,6 +1888,8 @@ int clv_set_schedule(clv_ctx *ctx, int32_t schedule) {
       return fail(ctx, CLV_ERR_ARG, "clv_set_schedule: the fused schedule does not fit LDS for shape " +
                                         std::to_string(b.X) + "x" + std::to_string(b.R) + "x" +
                                         std::to_string(b.Nt) + " at this precision");
+    if (schedule == 2 && b.kt_on)
+      return fail(ctx, CLV_ERR_ARG, "clv_set_schedule: chains with kin_dims = 3 run the two-pass schedule");
   }
   for (Bucket &b : ctx->buckets) b.schedule = (schedule == 0) ? (b.fused_lds ? 2 : 1) : schedule;
   drop_graphs(ctx);
@@ -1717,7 +1909,7 @@ int clv_set_profiling(clv_ctx *ctx, int32_t on) {
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   drain_events(ctx);
   ctx->profiling = on != 0;
-  for (int k = 0; k < 6; ++k) {
+  for (int k = 0; k < 7; ++k) {
     ctx->kt_ms[k] = 0.0;
     ctx->kt_n[k] = 0;
   }
@@ -1726,7 +1918,7 @@ int clv_set_profiling(clv_ctx *ctx, int32_t on) {
 
 int clv_kernel_time(clv_ctx *ctx, int32_t kind, double *total_ms, int64_t *launches) {
   if (!ctx) return CLV_ERR_ARG;
-  if (kind < 0 || kind > 5) return fail(ctx, CLV_ERR_ARG, "clv_kernel_time: bad kind");
+  if (kind < 0 || kind > 6) return fail(ctx, CLV_ERR_ARG, "clv_kernel_time: bad kind");
   (void)hipSetDevice(ctx->device);
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   drain_events(ctx);
@@ -1737,7 +1929,7 @@ int clv_kernel_time(clv_ctx *ctx, int32_t kind, double *total_ms, int64_t *launc
 
 int clv_get_schedule(const clv_ctx *ctx, int32_t chain) {
   if (!ctx || chain < 0 || chain >= int(ctx->where.size())) return CLV_ERR_ARG;
-  return ctx->buckets[ctx->where[chain].first].schedule;
+  return uses_fused(ctx->buckets[ctx->where[chain].first]) ? 2 : 1;
 }
 
 }  // extern "C"

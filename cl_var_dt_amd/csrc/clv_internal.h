@@ -45,9 +45,21 @@ struct Bucket {
   bool primed = false;   // current partials describe the current fields
   size_t part_slot = 0;  // doubles per partials slot
   size_t fused_lds = 0;  // LDS bytes of the fused kernel (0: not available)
+  // kin_dims = 3: transverse kinetic term kt (allocated when a chain asks for it)
+  struct KtAxis {
+    int L = 1, bmul = 0, rstride = 0, nbase = 0, W = 1;
+    void *tw = nullptr;
+    double *p2 = nullptr;
+  };
+  std::vector<KtAxis> kt_axes;  // y and/or z, the axes longer than 1
+  void *kt[2] = {nullptr, nullptr};
+  bool kt_on = false;
   std::vector<void *> allocs;
   int64_t bytes = 0;
 };
+
+// The fused schedule needs no transverse term (it would need it of the new fields).
+inline bool uses_fused(const Bucket &b) { return b.schedule == 2 && b.fused_lds > 0 && !b.kt_on; }
 
 }  // namespace clvi
 
@@ -65,8 +77,8 @@ struct clv_ctx {
   struct Pending { int kind; hipEvent_t a, b; };
   std::vector<Pending> pending;
   std::vector<hipEvent_t> event_pool;
-  double kt_ms[6] = {0, 0, 0, 0, 0, 0};
-  int64_t kt_n[6] = {0, 0, 0, 0, 0, 0};
+  double kt_ms[7] = {0, 0, 0, 0, 0, 0, 0};
+  int64_t kt_n[7] = {0, 0, 0, 0, 0, 0, 0};
   // captured two-step graphs of clv_propagate, keyed by the eval cadence and
   // the buckets' schedules and state-slot parities at capture time
   bool use_graphs = true;

@@ -21,6 +21,7 @@ struct BucketArgs {
   ChainState *st_next;          // written by the fused kernel (== st otherwise)
   const C2<T> *tw;              // 2X row-transform twiddles (RowFFT order)
   const double *p2;             // p2[k] = -p_x(k)^2 / X (FP64, from the host)
+  C2<T> *kt_psi, *kt_psib;      // transverse kinetic term (kin_dims = 3), or null
   double *part_rd;              // [chain][Nt][ntiles][2] (max, sum) of the current fields
   double *part_wr;              // partials written by this launch
   double *obs_stage;            // [chain][nchunks][Vs][3]
@@ -91,6 +92,7 @@ struct Coef {
   T af;       // a_x factor
   T cs[4];    // a_x factor eps s_i,  s = (mu+P+Q, mu, mu, mu-P+Q)
   T kA, kB;   // a_x factor eps (2G + c1),  a_x factor eps (2G + 2 c1)
+  T kK;       // a_x factor eps (weight of the unscaled transverse kinetic term)
   int q3;
 };
 
@@ -105,8 +107,22 @@ __device__ __forceinline__ Coef<T> coef(const ChainParams &p) {
   c.cs[3] = T(e * (p.mu - p.P + p.Q));
   c.kA = T(e * (2.0 * p.G + p.c1));
   c.kB = T(e * (2.0 * p.G + 2.0 * p.c1));
+  c.kK = T(e);
   c.q3 = p.q3_mode;
   return c;
+}
+
+// Does this chain add the transverse kinetic term (kin_dims = 3 on a lattice
+// with Y or Z > 1)?
+template <typename T>
+__device__ __forceinline__ bool has_kt(const BucketArgs<T> &A, const ChainParams &p) {
+  return A.kt_psi != nullptr && p.kin_dims == 3;
+}
+
+// k + kK * kt
+template <typename T>
+__device__ __forceinline__ C2<T> add_kt(C2<T> k, T kK, C2<T> kt) {
+  return {k.x + kK * kt.x, k.y + kK * kt.y};
 }
 
 // Multiplier-table scale of the drift kernels: a_x factor eps / a_x.

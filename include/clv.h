@@ -68,7 +68,8 @@ typedef struct clv_params {
   double alpha;        /* blend, 0.9 (Propagator_without_cutoff.cu:402)     */
   double factor;       /* drift prefactor, 1 in propagate (:427)            */
   uint64_t seed;       /* Philox key of this chain's noise stream           */
-  int32_t kin_dims;    /* 1: p^2 = p_x^2 (reference, :196); 3: px^2+py^2+pz^2 */
+  int32_t kin_dims;    /* 1: p^2 = p_x^2 (reference, :196); 3: px^2+py^2+pz^2 (two-pass
+                          schedule; Y, Z in the supported lengths; not in the server) */
   int32_t q3_mode;     /* CLV_Q3_REFERENCE or CLV_Q3_PSIBAR                 */
   int32_t noise_on;    /* 1: propagate (:406); 0: propagate_without_noise (:444) */
   int32_t reserved;
@@ -184,6 +185,7 @@ int clv_autotune(clv_ctx *ctx, const char *cache_path);
 /* Force a schedule: 0 = auto, 1 = two-pass (reduce then update),
  * 2 = fused single pass (update + next step's reduction). */
 int clv_set_schedule(clv_ctx *ctx, int32_t schedule);
+/* The schedule the chain's bucket runs (1 or 2). */
 int clv_get_schedule(const clv_ctx *ctx, int32_t chain);
 /* clv_propagate replays a captured HIP graph of two steps of every bucket
  * (on by default); off runs every step eagerly.  Results are identical. */
@@ -193,7 +195,8 @@ int clv_set_graphs(clv_ctx *ctx, int32_t on);
  * With profiling on, every kernel the context launches is bracketed by HIP
  * events on the context's stream; clv_kernel_time returns the accumulated
  * device time (ms) and launch count of one kernel kind:
- * 0 drift_stats, 1 controller, 2 update, 3 obs, 4 obs_finish, 5 fused step.
+ * 0 drift_stats, 1 controller, 2 update, 3 obs, 4 obs_finish, 5 fused step,
+ * 6 transverse kinetic term (kin_dims = 3).
  * (No counterpart in the reference, which times only the whole run, main.cu:59,159.) */
 int clv_set_profiling(clv_ctx *ctx, int32_t on);
 int clv_kernel_time(clv_ctx *ctx, int32_t kind, double *total_ms, int64_t *launches);

@@ -62,7 +62,7 @@ def test_errors_are_codes_not_faults():
     with pytest.raises(ClvError, match="out of range"):
         e.set_params(5, _p(1))
     with pytest.raises(ClvError, match="kin_dims"):
-        e.set_params(0, _p(1, kin_dims=3))
+        e.set_params(0, _p(1, kin_dims=2))
     with pytest.raises(ClvError):
         e.apply_update(0, -1.0)
     with pytest.raises(ClvError):

@@ -16,7 +16,9 @@ from cl_var_dt_amd import CLV_FP32, CLV_FP64, CLV_Q3_PSIBAR, Shape, default_para
 pytestmark = pytest.mark.gpu
 
 SHAPES = [(64, 1, 1, 32), (16, 16, 1, 8), (8, 4, 4, 4), (24, 6, 1, 4), (128, 2, 1, 4),
-          (256, 1, 1, 4), (32, 8, 1, 16), (48, 3, 1, 6), (4, 2, 1, 8)]
+          (256, 1, 1, 4), (32, 8, 1, 16), (48, 3, 1, 6), (4, 2, 1, 8),
+          # short rows with many of them: tiles limited by the update kernel's LDS
+          (4, 16, 8, 4), (8, 32, 1, 4)]
 
 
 def _engine(shapes, precision=CLV_FP64):
@@ -75,6 +77,33 @@ def test_drift_stats_and_one_step(oracle_lib, shape):
     c, d = orc.get_fields()
     assert np.max(abs(a - c)) < 1e-12 and np.max(abs(b - d)) < 1e-12
     eng.close()
+
+
+@pytest.mark.parametrize("shape", [(4, 64, 2, 4), (8, 32, 1, 4), (16, 16, 1, 4)])
+def test_short_rows_fp32(oracle_lib, shape):
+    """FP32 short-row shapes with many rows per tile: two steps against the
+    oracle (1e-5 relative), both schedules."""
+    print("case: short rows fp32", shape, flush=True)
+    from cl_var_dt_amd._abi import ClvError, SCHED_FUSED, SCHED_TWO_PASS
+    p = _params()
+    orc = oracle_lib.Oracle(Shape(*shape), p, nthreads=4)
+    orc.init_fields(1.0, 0.1)
+    orc.propagate(2)
+    c, d = orc.get_fields()
+    scale = np.sqrt(np.mean(abs(c) ** 2))
+    for sched in (SCHED_TWO_PASS, SCHED_FUSED):
+        eng = _engine([shape], CLV_FP32)
+        eng.set_params(0, p)
+        eng.init_fields(0, 1.0, 0.1)
+        try:
+            eng.set_schedule(sched)
+        except ClvError:  # fused does not fit this shape
+            eng.close()
+            continue
+        eng.propagate(2)
+        a, b = eng.get_fields(0)
+        assert max(np.max(abs(a - c)), np.max(abs(b - d))) / scale < 1e-5, sched
+        eng.close()
 
 
 def test_noise_free_update_matches(oracle_lib):

@@ -120,6 +120,19 @@ def test_kinetic_plane_waves(oracle_lib, dims):
         o.set_fields(f, f)
         K, _ = o.kinetic()
         assert np.allclose(K, -(2 * np.pi / Y) ** 2 * f / 0.7, atol=1e-12)
+    if Z > 1:
+        z = np.arange(Z)[None, None, :]
+        wave = np.exp(2j * np.pi * z / Z) * np.exp(2j * np.pi * x / X) * np.ones((X, Y, Z))
+        f = np.broadcast_to(wave, (4, Nt, X, Y, Z)).copy()
+        o.set_params(default_params(a_x=0.7, kin_dims=3))
+        o.set_fields(f, f)
+        K, _ = o.kinetic()
+        p2 = ph.momenta(X)[1] ** 2 + ph.momenta(Z)[1] ** 2
+        assert np.allclose(K, -p2 * f / 0.7, atol=1e-12)
+        o.set_params(default_params(a_x=0.7, kin_dims=1))
+        o.set_fields(f, f)
+        K, _ = o.kinetic()
+        assert np.allclose(K, -ph.momenta(X)[1] ** 2 * f / 0.7, atol=1e-12)
 
 
 def test_kinetic_matches_numpy(oracle_lib):
