@@ -1082,7 +1082,7 @@ int setup_bucket(clv_ctx *ctx, Bucket &b) {
   const size_t lds = size_t(b.esz) * 2 * b.X + size_t(b.esz / 2) * (b.X + (b.X & 1)) +
                      size_t(b.esz) * 9 * b.rows * row_pad(b.X, split_n2(b.X)) + sizeof(double) * 64;
   b.fused_lds = (lds <= 65536) ? lds : 0;
-  b.schedule = 1;
+  b.schedule = auto_schedule(b, b.nch);
   return CLV_OK;
 }
 
@@ -1891,7 +1891,7 @@ int clv_set_schedule(clv_ctx *ctx, int32_t schedule) {
     if (schedule == 2 && b.kt_on)
       return fail(ctx, CLV_ERR_ARG, "clv_set_schedule: chains with kin_dims = 3 run the two-pass schedule");
   }
-  for (Bucket &b : ctx->buckets) b.schedule = (schedule == 0) ? (b.fused_lds ? 2 : 1) : schedule;
+  for (Bucket &b : ctx->buckets) b.schedule = (schedule == 0) ? auto_schedule(b, b.nch) : schedule;
   drop_graphs(ctx);
   return CLV_OK;
 }

@@ -61,6 +61,16 @@ struct Bucket {
 // The fused schedule needs no transverse term (it would need it of the new fields).
 inline bool uses_fused(const Bucket &b) { return b.schedule == 2 && b.fused_lds > 0 && !b.kt_on; }
 
+// Schedule without timing: the fused march has one workgroup per (tile, chain)
+// and walks tau serially, so it only pays off with many of them; below that the
+// two-pass kernels (one workgroup per tile and tau) fill the GPU better.  The
+// threshold is measured on MI355X (fused ahead from ~4k workgroups).  Both
+// schedules give identical results, so the choice may change at any step.
+constexpr int kFusedMinWorkgroups = 4096;
+inline int auto_schedule(const Bucket &b, int nactive) {
+  return (b.fused_lds > 0 && !b.kt_on && int64_t(b.ntiles) * nactive >= kFusedMinWorkgroups) ? 2 : 1;
+}
+
 }  // namespace clvi
 
 struct clv_ctx {

@@ -105,7 +105,7 @@ Pool *make_pool(clv_server *s, const clv_shape &shape) {
     s->err = p->ctx.err;
     return nullptr;
   }
-  b.schedule = b.fused_lds ? 2 : 1;
+  b.schedule = 1;
   b.host_active.assign(b.nch, 0);
   // slots start inactive: mark every chain inactive on the device
   clv_params dflt;
@@ -276,6 +276,11 @@ void worker_main(clv_server *s) {
     }
     if (!any) continue;
     if (busy_since < 0) busy_since = now_s();
+    for (auto &pp : s->pools)
+      if (pp->alive && pp->nactive > 0) {
+        Bucket &b = pp->ctx.buckets[0];
+        b.schedule = auto_schedule(b, pp->nactive);
+      }
     for (int64_t n = 0; n < k; ++n)
       for (auto &pp : s->pools)
         if (pp->alive && pp->nactive > 0) step_bucket(&pp->ctx, pp->ctx.buckets[0], -1);
