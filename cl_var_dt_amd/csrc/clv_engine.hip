@@ -257,7 +257,7 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? CLV_FUSED_WAVES_PER_EU : 2) v
     return;
   }
   const int rows = A.rows, Nt = A.Nt;
-  Smem<T, X> sm(smem_raw, 9 * rows);
+  Smem<T, X> sm(smem_raw, 9 * A.fsets * rows);  // fsets sets of 9 row sets
   using RF = RowFFT<T, X>;
   load_tables<T, X>(sm, A.tw, A.p2, drift_table_scale(P), tid, nthr);
   if (tid < 64) {
@@ -325,8 +325,26 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? CLV_FUSED_WAVES_PER_EU : 2) v
     for (int c = 0; c < 4; ++c) km[c] = sm.at(c * rows + j, x);
   }
   __syncthreads();  // the LDS lines are rewritten below
+  // With two LDS sets the transform lines alternate by tau parity, so a tau's
+  // lines can be written while slower waves still read the previous tau's
+  // results (one set: a barrier ends each tau).  Each tau's (max, sum) of
+  // |D_0| is combined across the waves after the next tau's first barrier, in
+  // the order of block_max_sum.
+  const int w = tid >> 6, nw = (nthr + 63) >> 6;
+  auto commit_partial = [&](int tt) {  // thread 0, after a barrier
+    const double *rm = sm.red + (tt & 1) * 16;
+    double mm = rm[0], ss = rm[8];
+    for (int q = 1; q < nw; ++q) {
+      mm = fmax(mm, rm[q]);
+      ss += rm[8 + q];
+    }
+    double *o = A.part_wr + ((int64_t(chain) * Nt + tt) * A.ntiles + tile) * 2;
+    o[0] = mm;
+    o[1] = ss;
+  };
   for (int t = 0; t < Nt; ++t) {
     const int tp1 = (t == Nt - 1) ? 0 : t + 1;
+    const int L0 = (t & (A.fsets - 1)) * 9 * rows;  // first line of this tau's set
     C2<T> pp[4], bn[4];
     if (has_site) {
       load4(psi, site_at(tp1), pp);
@@ -334,22 +352,23 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? CLV_FUSED_WAVES_PER_EU : 2) v
       if (t + 2 <= Nt) load4(psib, site_at(t + 2 == Nt ? 0 : t + 2), bn);
 #pragma unroll
       for (int c = 0; c < 4; ++c) {
-        sm.at(c * rows + j, x) = p0[c];
-        sm.at((4 + c) * rows + j, x) = bp[c];
+        sm.at(L0 + c * rows + j, x) = p0[c];
+        sm.at(L0 + (4 + c) * rows + j, x) = bp[c];
       }
-      sm.at(8 * rows + j, x) = nprev[0];
+      sm.at(L0 + 8 * rows + j, x) = nprev[0];
     }
     __syncthreads();
-    RF::kinetic(sm.buf, sm.tw, sm.mult, 9 * rows, tid, nthr);
+    if (tid == 0 && t > 1) commit_partial(t - 1);
+    RF::kinetic(sm.buf + L0 * RF::PAD, sm.tw, sm.mult, 9 * rows, tid, nthr);
     double v = 0.0;
     if (has_site) {
       C2<T> kn[4], kp[4];
 #pragma unroll
       for (int c = 0; c < 4; ++c) {
-        kn[c] = sm.at(c * rows + j, x);
-        kp[c] = sm.at((4 + c) * rows + j, x);
+        kn[c] = sm.at(L0 + c * rows + j, x);
+        kp[c] = sm.at(L0 + (4 + c) * rows + j, x);
       }
-      const C2<T> k0n = sm.at(8 * rows + j, x);
+      const C2<T> k0n = sm.at(L0 + 8 * rows + j, x);
       C2<T> D[4], Db[4];
       drift_site(k, pm, p0, pp, b0, bp, km, kp, D, Db);
       T nz[8] = {};
@@ -384,17 +403,16 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? CLV_FUSED_WAVES_PER_EU : 2) v
       }
     }
     if (t > 0) {
-      double m, s;
-      block_max_sum(v, sm.red, tid, nthr, &m, &s);
-      if (tid == 0) {
-        double *o = A.part_wr + ((int64_t(chain) * Nt + t) * A.ntiles + tile) * 2;
-        o[0] = m;
-        o[1] = s;
+      const double wm = wave_max(v), ws = wave_sum(v);
+      if ((tid & 63) == 0) {
+        sm.red[(t & 1) * 16 + w] = wm;
+        sm.red[(t & 1) * 16 + 8 + w] = ws;
       }
-    } else {
-      __syncthreads();  // LDS lines are rewritten by the next tau
     }
+    if (A.fsets == 1) __syncthreads();  // the one set is rewritten by the next tau
   }
+  __syncthreads();
+  if (tid == 0 && Nt > 1) commit_partial(Nt - 1);
   // |D_0| of the new tau = 0 (needs the new psi(Nt-1)); this thread wrote
   // the new tau = 0 values of its site at t = 0
   C2<T> n0k[4], nb0k[4];
@@ -686,6 +704,7 @@ BucketArgs<T> make_args(const Bucket &b, int chain0, int eval_every, bool fused 
   A.nchunks = b.nchunks;
   A.tc = b.tc;
   A.tcs = b.tcs;
+  A.fsets = b.fused_sets;
   A.chain0 = chain0;
   A.eval_every = eval_every;
   return A;
@@ -711,7 +730,7 @@ void launch_row_kernels(const Bucket &b, hipStream_t s, int which, int chain0, i
     hipLaunchKernelGGL((k_obs<T, X>), dim3(b.ntiles, b.nchunks, nch), block, sh, s, A);
   } else {
     BucketArgs<T> A = make_args<T>(b, chain0, eval_every, true);
-    const size_t sh = Smem<T, X>::bytes(9 * b.rows);
+    const size_t sh = Smem<T, X>::bytes(9 * b.fused_sets * b.rows);
     hipLaunchKernelGGL((k_fused<T, X>), dim3(b.ntiles, nch), block, sh, s, A);
   }
 }
@@ -1079,9 +1098,12 @@ int setup_bucket(clv_ctx *ctx, Bucket &b) {
   b.host_eval.assign(b.nch, 0);
   b.host_eval_from.assign(b.nch, 0);
   b.host_active.assign(b.nch, 1);
-  const size_t lds = size_t(b.esz) * 2 * b.X + size_t(b.esz / 2) * (b.X + (b.X & 1)) +
-                     size_t(b.esz) * 9 * b.rows * row_pad(b.X, split_n2(b.X)) + sizeof(double) * 64;
-  b.fused_lds = (lds <= 65536) ? lds : 0;
+  const auto fused_lds = [&](int sets) {
+    return size_t(b.esz) * 2 * b.X + size_t(b.esz / 2) * (b.X + (b.X & 1)) +
+           size_t(b.esz) * 9 * sets * b.rows * row_pad(b.X, split_n2(b.X)) + sizeof(double) * 64;
+  };
+  b.fused_sets = (fused_lds(2) <= 65536) ? 2 : 1;
+  b.fused_lds = (fused_lds(b.fused_sets) <= 65536) ? fused_lds(b.fused_sets) : 0;
   b.schedule = auto_schedule(b, b.nch);
   return CLV_OK;
 }

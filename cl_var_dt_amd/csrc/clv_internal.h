@@ -45,6 +45,7 @@ struct Bucket {
   bool primed = false;   // current partials describe the current fields
   size_t part_slot = 0;  // doubles per partials slot
   size_t fused_lds = 0;  // LDS bytes of the fused kernel (0: not available)
+  int fused_sets = 1;    // LDS line sets of the fused kernel (2 when they fit)
   // kin_dims = 3: transverse kinetic term kt (allocated when a chain asks for it)
   struct KtAxis {
     int L = 1, bmul = 0, rstride = 0, nbase = 0, W = 1;

@@ -33,6 +33,7 @@ struct BucketArgs {
   int64_t S, Vs;
   int R, Nt, rows, ntiles, nchunks, tc;
   int tcs;                      // tau slices per drift-stats workgroup
+  int fsets;                    // fused kernel: LDS line sets (2: alternate by tau parity)
   int chain0;                   // first chain of the launch
   int eval_every;               // obs kernels: evaluate chains with cur_step % eval_every == 0;
                                 // < 0: each chain's own ChainParams::eval_every; 0: always
