@@ -258,9 +258,22 @@ __device__ __forceinline__ void load_tables(Smem<T, X> &sm, const C2<T> *tw, con
 __device__ __forceinline__ void reduce_partials(const double *pp, int n, int lane, double *m,
                                                 double *s) {
   double mm = 0.0, ss = 0.0;
-  for (int i = lane; i < n; i += 64) {
-    mm = fmax(mm, pp[2 * i]);
-    ss += pp[2 * i + 1];
+  const double2 *p2 = reinterpret_cast<const double2 *>(pp);
+  int i = lane;
+  // eight loads in flight, then the same sequential fold per lane
+  for (; i + 7 * 64 < n; i += 8 * 64) {
+    double2 v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = p2[i + u * 64];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      mm = fmax(mm, v[u].x);
+      ss += v[u].y;
+    }
+  }
+  for (; i < n; i += 64) {
+    mm = fmax(mm, p2[i].x);
+    ss += p2[i].y;
   }
   *m = wave_max(mm);
   *s = wave_sum(ss);
