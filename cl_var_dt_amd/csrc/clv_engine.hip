@@ -462,16 +462,29 @@ __global__ __launch_bounds__(256) void k_obs(BucketArgs<T> A) {
   const int r = tile * A.rows + j;
   const double ax = P.a_x, G = P.G, c1 = P.c1;
   const double sz[4] = {P.P + P.Q, 0.0, 0.0, -P.P + P.Q};
+  // interaction part of h through the matrix invariants of the pair
+  // (psibar(t+1), psi(t)): G tr[(Psibar^T Psi)^2] + (c1/2) sum_a F_a^2
+  //   = (G + c1/2) N^2 - (2G + 2c1) det Psibar det Psi   (N = sum_c psib_c psi_c)
+  const double gN = G + 0.5 * c1, gD = 2.0 * G + 2.0 * c1;
   double n[4] = {0, 0, 0, 0}, e = 0.0, F[3] = {0, 0, 0};
+  // the slice's fields are loaded one slice ahead, overlapping the transform
+  auto site_at = [&](int t) { return (int64_t(t) * A.R + r) * X + x; };
+  C2<T> q[4], b[4];
+  if (has_site) {
+    const int t = chunk * A.tc;
+    load4(psi, site_at(t), q);
+    load4(psib, site_at(t == A.Nt - 1 ? 0 : t + 1), b);
+  }
   for (int tt = 0; tt < A.tc; ++tt) {
     const int t = chunk * A.tc + tt;
-    const int tp1 = (t == A.Nt - 1) ? 0 : t + 1;
-    C2<T> q[4], b[4];
+    C2<T> qn[4], bn[4];
     if (has_site) {
-      load4(psi, (int64_t(t) * A.R + r) * X + x, q);
-      load4(psib, (int64_t(tp1) * A.R + r) * X + x, b);
 #pragma unroll
       for (int c = 0; c < 4; ++c) sm.at(c * A.rows + j, x) = q[c];
+      if (tt + 1 < A.tc) {
+        load4(psi, site_at(t + 1), qn);
+        load4(psib, site_at(t + 2 >= A.Nt ? t + 2 - A.Nt : t + 2), bn);
+      }
     }
     __syncthreads();
     RowFFT<T, X>::kinetic(sm.buf, sm.tw, sm.mult, 4 * A.rows, tid, nthr);
@@ -484,37 +497,39 @@ __global__ __launch_bounds__(256) void k_obs(BucketArgs<T> A) {
         qq[c] = {double(q[c].x), double(q[c].y)};
         kk[c] = {double(kc.x), double(kc.y)};
         if (has_kt(A, P)) {
-          const C2<T> kt = load1(A.kt_psi + chain * A.chain_stride, (int64_t(t) * A.R + r) * X + x, c);
+          const C2<T> kt = load1(A.kt_psi + chain * A.chain_stride, site_at(t), c);
           kk[c].x += double(kt.x);
           kk[c].y += double(kt.y);
         }
       }
-      C2<double> h = {0.0, 0.0}, nn = {0.0, 0.0};
+      // only Re h is accumulated
+      C2<double> nn = {0.0, 0.0};
+      double hr = 0.0, bq_re[4];
 #pragma unroll
       for (int c = 0; c < 4; ++c) {
         const C2<double> bq = cmul(bb[c], qq[c]);
+        bq_re[c] = bq.x;
         n[c] += bq.x * ax;
         nn = cadd(nn, bq);
-        h = csub(h, cadd(cmul(bb[c], kk[c]), cscale(sz[c], bq)));
+        hr -= (bb[c].x * kk[c].x - bb[c].y * kk[c].y) + sz[c] * bq.x;
       }
       const C2<double> detb = csub(cmul(bb[0], bb[3]), cmul(bb[1], bb[2]));
       const C2<double> detq = csub(cmul(qq[0], qq[3]), cmul(qq[1], qq[2]));
-      h = cadd(h, cscale(G, csub(cmul(nn, nn), cscale(2.0, cmul(detb, detq)))));
-      const C2<double> Fx = cadd(cadd(cmul(bb[2], qq[0]), cmul(bb[0], qq[2])),
-                                 cadd(cmul(bb[3], qq[1]), cmul(bb[1], qq[3])));
-      const C2<double> Fyi = csub(cadd(cmul(bb[2], qq[0]), cmul(bb[3], qq[1])),
-                                  cadd(cmul(bb[0], qq[2]), cmul(bb[1], qq[3])));
-      const C2<double> Fy = {-Fyi.y, Fyi.x};  // i * Fyi
-      const C2<double> Fz = csub(cadd(cmul(bb[0], qq[0]), cmul(bb[1], qq[1])),
-                                 cadd(cmul(bb[2], qq[2]), cmul(bb[3], qq[3])));
-      const C2<double> F2 = cadd(cadd(cmul(Fx, Fx), cmul(Fy, Fy)), cmul(Fz, Fz));
-      h = cadd(h, cscale(0.5 * c1, F2));
-      e += h.x * ax;
-      F[0] += Fx.x;
-      F[1] += Fy.x;
-      F[2] += Fz.x;
+      hr += gN * (nn.x * nn.x - nn.y * nn.y) - gD * (detb.x * detq.x - detb.y * detq.y);
+      e += hr * ax;
+      // Re F_x, Re F_y = -Im(psib_2 psi_0 + psib_3 psi_1 - psib_0 psi_2 - psib_1 psi_3), Re F_z
+      F[0] += (bb[2].x * qq[0].x - bb[2].y * qq[0].y) + (bb[0].x * qq[2].x - bb[0].y * qq[2].y) +
+              (bb[3].x * qq[1].x - bb[3].y * qq[1].y) + (bb[1].x * qq[3].x - bb[1].y * qq[3].y);
+      F[1] -= (bb[2].x * qq[0].y + bb[2].y * qq[0].x) + (bb[3].x * qq[1].y + bb[3].y * qq[1].x) -
+              (bb[0].x * qq[2].y + bb[0].y * qq[2].x) - (bb[1].x * qq[3].y + bb[1].y * qq[3].x);
+      F[2] += (bq_re[0] + bq_re[1]) - (bq_re[2] + bq_re[3]);
     }
     __syncthreads();
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      q[c] = qn[c];
+      b[c] = bn[c];
+    }
   }
   if (has_site) {
     const int64_t vs = int64_t(x) * A.R + r;  // reference spatial order (x, y, z)
