@@ -1,14 +1,16 @@
 // clv_server.hip -- the serving loop: continuous batching of Langevin runs.
 //
-// A worker thread owns the device.  Runs wait in a FIFO queue; at every tick
-// boundary the worker admits as many as fit into free slots of the pool of
-// their shape, creating a pool when the HBM budget allows (and releasing
-// empty pools of other shapes when it does not).  Each pool is a bucket of
-// equal-shape chain slots with its own HIP stream, so pools of different
-// shapes run concurrently on the GPU.  A tick advances every active slot by
-// the smallest remaining step count (capped), so every run stops exactly
-// after its own nsteps; finished runs are read back, their slots freed, and
-// waiting clients woken.  Nothing a run computes depends on its slot or on
+// A worker thread owns the device.  Runs wait in a FIFO queue; the worker
+// admits them into free slots of an idle pool of their shape, creating a pool
+// when the HBM budget allows (and releasing empty pools of other shapes when
+// it does not).  Each pool is a bucket of equal-shape chain slots with its own
+// HIP stream, so pools of different shapes run concurrently on the GPU.  Pools
+// tick independently: an idle pool with active slots is given k steps (the
+// smallest remaining step count of its runs, capped so that a tick lasts about
+// kTickTargetMs), the worker polls the tick's completion event, and a pool
+// whose tick ended is accounted, its finished runs read back and freed, and
+// it is refilled and relaunched without waiting for the other pools.  Every
+// run stops exactly after its own nsteps.  Nothing a run computes depends on its slot or on
 // its neighbours: noise is keyed by the run's seed and step, reductions are
 // per chain and in a fixed order.
 #include <hip/hip_runtime.h>
@@ -50,13 +52,19 @@ struct Pool {
   std::vector<Run *> slot_run;  // run per slot, nullptr = free
   int nactive = 0;
   bool alive = false;
+  bool busy = false;            // a tick is in flight on the stream
+  int64_t k_inflight = 0;       // steps of that tick
+  double ms_per_step = 0.0;     // measured by the last tick (0: unknown)
+  hipEvent_t ev_start = nullptr, ev_done = nullptr;
 };
 
 bool same_shape(const clv_shape &a, const clv_shape &b) {
   return a.nx == b.nx && a.ny == b.ny && a.nz == b.nz && a.nt == b.nt;
 }
 
-constexpr int64_t kTickMax = 25;           // steps per tick at most
+constexpr int64_t kTickMax = 100;          // steps per tick at most
+constexpr double kTickTargetMs = 2.0;      // aimed tick duration
+constexpr int64_t kTickFirst = 10;         // steps of a pool's first tick
 constexpr int64_t kPoolTargetBytes = 4LL << 30;
 constexpr int kPoolMaxSlots = 64;
 
@@ -112,6 +120,12 @@ Pool *make_pool(clv_server *s, const clv_shape &shape) {
   clv_default_params(&dflt);
   for (int i = 0; i < b.nch; ++i) (void)chain_set_active(&p->ctx, b, i, dflt, 0, false);
   p->slot_run.assign(cap, nullptr);
+  if (hipEventCreate(&p->ev_start) != hipSuccess || hipEventCreate(&p->ev_done) != hipSuccess) {
+    free_bucket(&p->ctx, b);
+    (void)hipStreamDestroy(p->ctx.stream);
+    s->err = "clv_server: hipEventCreate failed";
+    return nullptr;
+  }
   p->alive = true;
   s->bytes_in_use += pool_bytes(*p);
   s->pools.push_back(std::move(p));
@@ -124,16 +138,26 @@ void free_pool(clv_server *s, Pool &p) {
   s->bytes_in_use -= pool_bytes(p);
   for (Bucket &b : p.ctx.buckets) free_bucket(&p.ctx, b);
   (void)hipStreamDestroy(p.ctx.stream);
+  if (p.ev_start) (void)hipEventDestroy(p.ev_start);
+  if (p.ev_done) (void)hipEventDestroy(p.ev_done);
+  p.ev_start = p.ev_done = nullptr;
   p.ctx.buckets.clear();
   p.alive = false;
+  p.busy = false;
 }
 
-// Find a free slot for `shape` (existing pool, new pool, or after releasing
-// empty pools of other shapes).  Returns false if it has to wait.
+// Find a free slot for `shape` (idle pool of that shape, new pool, or after
+// releasing empty pools of other shapes).  Returns false if it has to wait;
+// a pool with a tick in flight takes new runs when the tick has ended.
 bool place(clv_server *s, const clv_shape &shape, int *pool, int *slot) {
+  bool shape_busy = false;
   for (size_t k = 0; k < s->pools.size(); ++k) {
     Pool &p = *s->pools[k];
     if (!p.alive || !same_shape(p.ctx.buckets[0].shape, shape)) continue;
+    if (p.busy) {
+      for (Run *r : p.slot_run) shape_busy |= (r == nullptr);
+      continue;
+    }
     for (size_t i = 0; i < p.slot_run.size(); ++i)
       if (p.slot_run[i] == nullptr) {
         *pool = int(k);
@@ -141,12 +165,13 @@ bool place(clv_server *s, const clv_shape &shape, int *pool, int *slot) {
         return true;
       }
   }
+  if (shape_busy) return false;  // a slot frees up at the end of that pool's tick
   Pool *np = make_pool(s, shape);
   if (!np) {
     // release idle pools of other shapes, then try once more
     bool freed = false;
     for (auto &pp : s->pools)
-      if (pp->alive && pp->nactive == 0 && !same_shape(pp->ctx.buckets[0].shape, shape)) {
+      if (pp->alive && !pp->busy && pp->nactive == 0 && !same_shape(pp->ctx.buckets[0].shape, shape)) {
         free_pool(s, *pp);
         freed = true;
       }
@@ -263,44 +288,53 @@ void worker_main(clv_server *s) {
         finish_run(s, r);
       }
     }
-    // ---- one tick: every active slot advances k steps ----
-    int64_t k = kTickMax;
-    bool any = false;
+    // ---- launch a tick on every idle pool with active slots ----
+    bool launched = false;
     for (auto &pp : s->pools) {
-      if (!pp->alive || pp->nactive == 0) continue;
-      for (Run *r : pp->slot_run)
-        if (r) {
-          k = std::min(k, r->remaining);
-          any = true;
-        }
+      Pool &p = *pp;
+      if (!p.alive || p.busy || p.nactive == 0) continue;
+      int64_t k = p.ms_per_step > 0.0 ? int64_t(kTickTargetMs / p.ms_per_step) : kTickFirst;
+      k = std::max<int64_t>(1, std::min(k, kTickMax));
+      for (Run *r : p.slot_run)
+        if (r) k = std::min(k, r->remaining);
+      Bucket &b = p.ctx.buckets[0];
+      b.schedule = auto_schedule(b, p.nactive);
+      (void)hipEventRecord(p.ev_start, p.ctx.stream);
+      for (int64_t n = 0; n < k; ++n) step_bucket(&p.ctx, b, -1);
+      (void)hipEventRecord(p.ev_done, p.ctx.stream);
+      p.busy = true;
+      p.k_inflight = k;
+      launched = true;
     }
-    if (!any) continue;
-    if (busy_since < 0) busy_since = now_s();
-    for (auto &pp : s->pools)
-      if (pp->alive && pp->nactive > 0) {
-        Bucket &b = pp->ctx.buckets[0];
-        b.schedule = auto_schedule(b, pp->nactive);
-      }
-    for (int64_t n = 0; n < k; ++n)
-      for (auto &pp : s->pools)
-        if (pp->alive && pp->nactive > 0) step_bucket(&pp->ctx, pp->ctx.buckets[0], -1);
+    if (launched && busy_since < 0) busy_since = now_s();
+    // ---- account the pools whose tick has ended ----
+    bool progressed = false;
     double su = 0.0;
     std::vector<Run *> done;
     for (auto &pp : s->pools) {
-      if (!pp->alive || pp->nactive == 0) continue;
-      (void)hipStreamSynchronize(pp->ctx.stream);
-      su += double(pp->nactive) * double(pp->ctx.buckets[0].S) * double(k);
-      for (Run *r : pp->slot_run)
+      Pool &p = *pp;
+      if (!p.alive || !p.busy) continue;
+      const hipError_t q = hipEventQuery(p.ev_done);
+      if (q == hipErrorNotReady) continue;
+      float ms = 0.f;
+      if (q == hipSuccess && hipEventElapsedTime(&ms, p.ev_start, p.ev_done) == hipSuccess)
+        p.ms_per_step = double(ms) / double(p.k_inflight);
+      p.busy = false;
+      progressed = true;
+      su += double(p.nactive) * double(p.ctx.buckets[0].S) * double(p.k_inflight);
+      for (Run *r : p.slot_run)
         if (r) {
-          r->remaining -= k;
+          r->remaining -= p.k_inflight;
           if (r->remaining <= 0) done.push_back(r);
         }
     }
-    {
+    if (progressed) {
       std::lock_guard<std::mutex> g(s->mu);
       s->info.site_updates += su;
     }
     for (Run *r : done) finish_run(s, *r);
+    if (!progressed && !launched && admitted.empty())
+      std::this_thread::sleep_for(std::chrono::microseconds(50));
   }
   std::lock_guard<std::mutex> g(s->mu);
   if (busy_since >= 0) s->info.busy_seconds += now_s() - busy_since;
