@@ -1120,7 +1120,7 @@ int setup_bucket(clv_ctx *ctx, Bucket &b) {
     return size_t(b.esz) * 2 * b.X + size_t(b.esz / 2) * (b.X + (b.X & 1)) +
            size_t(b.esz) * 9 * sets * b.rows * row_pad(b.X, split_n2(b.X)) + sizeof(double) * 64;
   };
-  b.fused_sets = (fused_lds(2) <= 65536) ? 2 : 1;
+  b.fused_sets = b.fused_sets_max = (fused_lds(2) <= 65536) ? 2 : 1;
   b.fused_lds = (fused_lds(b.fused_sets) <= 65536) ? fused_lds(b.fused_sets) : 0;
   b.schedule = auto_schedule(b, b.nch);
   return CLV_OK;
@@ -1782,19 +1782,38 @@ std::string tune_key(const std::string &dev, const Bucket &b, int precision) {
   return k.str();
 }
 
-// cache lines: {"key": "...", "schedule": S, "ms_two_pass": a, "ms_fused": b}
-// (0 for a schedule that does not apply)
-std::map<std::string, int> read_tune_cache(const char *path) {
-  std::map<std::string, int> out;
+// One tuning choice.  All of them give identical results: the schedules are
+// bit-identical, the reduction tile (tcs tau slices per drift-statistics
+// workgroup) does not change the per-(tau, tile) partials, and the fused
+// kernel's LDS line sets only change its barriers.  The row tile and the
+// radix split are fixed per shape: they set the partial-sum granularity and
+// the transform's rounding, so tuning them would make results depend on a
+// timing measurement.
+struct TuneChoice {
+  int schedule = 1, tcs = 1, fsets = 1;
+};
+
+int json_int(const std::string &line, const char *field, int dflt) {
+  const std::string k = std::string("\"") + field + "\": ";
+  const size_t p = line.find(k);
+  return p == std::string::npos ? dflt : std::atoi(line.c_str() + p + k.size());
+}
+
+// cache lines: {"key": "...", "schedule": S, "tcs": T, "fsets": F, "ms": best, ...}
+std::map<std::string, TuneChoice> read_tune_cache(const char *path) {
+  std::map<std::string, TuneChoice> out;
   if (!path) return out;
   std::ifstream in(path);
   std::string line;
   while (std::getline(in, line)) {
     const size_t k0 = line.find("\"key\": \"");
-    const size_t s0 = line.find("\"schedule\": ");
-    if (k0 == std::string::npos || s0 == std::string::npos) continue;
+    if (k0 == std::string::npos || line.find("\"schedule\": ") == std::string::npos) continue;
     const size_t k1 = line.find('"', k0 + 8);
-    out[line.substr(k0 + 8, k1 - k0 - 8)] = std::atoi(line.c_str() + s0 + 12);
+    TuneChoice c;
+    c.schedule = json_int(line, "schedule", 1);
+    c.tcs = json_int(line, "tcs", 0);
+    c.fsets = json_int(line, "fsets", 0);
+    out[line.substr(k0 + 8, k1 - k0 - 8)] = c;
   }
   return out;
 }
@@ -1844,9 +1863,37 @@ int restore_snapshot(clv_ctx *ctx, Bucket &b, Snapshot &sn, bool release) {
   return CLV_OK;
 }
 
-// ms per step of bucket b under `schedule` (eager steps, no observables)
-int time_schedule(clv_ctx *ctx, Bucket &b, int schedule, int nsteps, double *ms) {
-  b.schedule = schedule;
+// Valid choices for bucket b (reduction tiles that divide Nt and fit LDS;
+// fused line sets that fit 64 KB).
+std::vector<TuneChoice> tune_choices(const Bucket &b, int fused_sets_max) {
+  std::vector<TuneChoice> out;
+  const size_t line = size_t(b.esz) * row_pad(b.X, split_n2(b.X));
+  for (int q = 1; q <= 4; ++q)
+    if (b.Nt % q == 0 && q * b.rows * b.X <= 1024 && line * q * b.rows <= 48 * 1024)
+      out.push_back({1, q, 1});
+  if (b.fused_lds > 0 && !b.kt_on)
+    for (int f = 1; f <= fused_sets_max; ++f) out.push_back({2, b.tcs, f});
+  return out;
+}
+
+bool choice_valid(const Bucket &b, const TuneChoice &c, int fused_sets_max) {
+  for (const TuneChoice &v : tune_choices(b, fused_sets_max))
+    if (v.schedule == c.schedule && (c.schedule == 2 || v.tcs == c.tcs) &&
+        (c.schedule == 1 || v.fsets == c.fsets))
+      return true;
+  return false;
+}
+
+void apply_choice(Bucket &b, const TuneChoice &c) {
+  b.schedule = c.schedule;
+  if (c.schedule == 1) b.tcs = c.tcs;
+  else b.fused_sets = c.fsets;
+  b.primed = false;
+}
+
+// ms per step of bucket b under choice c (eager steps, no observables)
+int time_choice(clv_ctx *ctx, Bucket &b, const TuneChoice &c, int nsteps, double *ms) {
+  apply_choice(b, c);
   for (int n = 0; n < 2; ++n) step_bucket(ctx, b, 0);
   hipEvent_t e0, e1;
   HIPCHK(ctx, hipEventCreate(&e0));
@@ -1863,10 +1910,6 @@ int time_schedule(clv_ctx *ctx, Bucket &b, int schedule, int nsteps, double *ms)
   return CLV_OK;
 }
 
-bool schedule_fits(const Bucket &b, int schedule) {
-  return schedule == 1 || (schedule == 2 && b.fused_lds > 0 && !b.kt_on);
-}
-
 }  // namespace
 
 int clv_autotune(clv_ctx *ctx, const char *cache_path) {
@@ -1874,45 +1917,50 @@ int clv_autotune(clv_ctx *ctx, const char *cache_path) {
   (void)hipSetDevice(ctx->device);
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   const std::string dev = device_name(ctx->device);
-  const std::map<std::string, int> cache = read_tune_cache(cache_path);
+  const std::map<std::string, TuneChoice> cache = read_tune_cache(cache_path);
   const bool prof = ctx->profiling;
   ctx->profiling = false;
   int rc = CLV_OK;
   for (Bucket &b : ctx->buckets) {
     const std::string key = tune_key(dev, b, ctx->precision) + "|" + std::to_string(b.nch) + "ch";
+    const int fsmax = b.fused_sets_max;
     auto hit = cache.find(key);
-    if (hit != cache.end() && hit->second >= 1 && hit->second <= 2 && schedule_fits(b, hit->second)) {
-      b.schedule = hit->second;
+    if (hit != cache.end() && choice_valid(b, hit->second, fsmax)) {
+      apply_choice(b, hit->second);
       continue;
     }
-    std::vector<int> cand;
-    for (int sc = 1; sc <= 2; ++sc)
-      if (schedule_fits(b, sc)) cand.push_back(sc);
-    if (cand.size() == 1) {
-      b.schedule = 1;
+    const std::vector<TuneChoice> cand = tune_choices(b, fsmax);
+    if (cand.size() <= 1) {
+      if (!cand.empty()) apply_choice(b, cand[0]);
       continue;
     }
     Snapshot sn;
     if ((rc = take_snapshot(ctx, b, sn))) break;
-    double ms[3] = {0, 0, 0};
+    std::vector<double> ms(cand.size(), 0.0);
     const int n = 8;
     for (size_t q = 0; q < cand.size() && !rc; ++q) {
       if (q > 0) rc = restore_snapshot(ctx, b, sn, false);
-      if (!rc) rc = time_schedule(ctx, b, cand[q], n, &ms[cand[q]]);
+      if (!rc) rc = time_choice(ctx, b, cand[q], n, &ms[q]);
     }
     const int rc2 = restore_snapshot(ctx, b, sn, true);
     if (rc || rc2) {
       rc = rc ? rc : rc2;
       break;
     }
-    int best = cand[0];
-    for (int sc : cand)
-      if (ms[sc] < ms[best]) best = sc;
-    b.schedule = best;
+    size_t best = 0;
+    for (size_t q = 1; q < cand.size(); ++q)
+      if (ms[q] < ms[best]) best = q;
+    apply_choice(b, cand[best]);
     if (cache_path) {
+      double two = 0.0, fused = 0.0;  // best of each schedule
+      for (size_t q = 0; q < cand.size(); ++q) {
+        double &m = cand[q].schedule == 1 ? two : fused;
+        if (m == 0.0 || ms[q] < m) m = ms[q];
+      }
       std::ofstream out(cache_path, std::ios::app);
-      out << "{\"key\": \"" << key << "\", \"schedule\": " << b.schedule
-          << ", \"ms_two_pass\": " << ms[1] << ", \"ms_fused\": " << ms[2] << "}\n";
+      out << "{\"key\": \"" << key << "\", \"schedule\": " << cand[best].schedule
+          << ", \"tcs\": " << b.tcs << ", \"fsets\": " << b.fused_sets << ", \"ms\": " << ms[best]
+          << ", \"ms_two_pass\": " << two << ", \"ms_fused\": " << fused << "}\n";
     }
   }
   ctx->profiling = prof;

@@ -46,6 +46,7 @@ struct Bucket {
   size_t part_slot = 0;  // doubles per partials slot
   size_t fused_lds = 0;  // LDS bytes of the fused kernel (0: not available)
   int fused_sets = 1;    // LDS line sets of the fused kernel (2 when they fit)
+  int fused_sets_max = 1;
   // kin_dims = 3: transverse kinetic term kt (allocated when a chain asks for it)
   struct KtAxis {
     int L = 1, bmul = 0, rstride = 0, nbase = 0, W = 1;
@@ -65,9 +66,9 @@ inline bool uses_fused(const Bucket &b) { return b.schedule == 2 && b.fused_lds 
 // Schedule without timing: the fused march has one workgroup per (tile, chain)
 // and walks tau serially, so it only pays off with many of them; below that the
 // two-pass kernels (one workgroup per tile and tau) fill the GPU better.  The
-// threshold is measured on MI355X (fused ahead from ~4k workgroups).  Both
-// schedules give identical results, so the choice may change at any step.
-constexpr int kFusedMinWorkgroups = 4096;
+// threshold is measured on MI355X (level at ~1k workgroups, fused ahead above).
+// Both schedules give identical results, so the choice may change at any step.
+constexpr int kFusedMinWorkgroups = 1024;
 inline int auto_schedule(const Bucket &b, int nactive) {
   return (b.fused_lds > 0 && !b.kt_on && int64_t(b.ntiles) * nactive >= kFusedMinWorkgroups) ? 2 : 1;
 }

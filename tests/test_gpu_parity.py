@@ -338,8 +338,9 @@ def test_graph_replay_equals_eager(sched):
 
 
 def test_autotune_keeps_state_and_caches(tmp_path):
-    """The autotuner times the schedules on a snapshot of the live state and
-    restores it; the choice is cached per (device, shape, precision)."""
+    """The autotuner times the schedules, reduction tiles and fused LDS sets on a
+    snapshot of the live state and restores it; the choice is cached per
+    (device, shape, precision, chain count) and changes no result."""
     print("case: autotune", flush=True)
     shapes = [(64, 4, 1, 8), (16, 8, 2, 4)]
     cache = str(tmp_path / "tune.jsonl")
@@ -358,7 +359,8 @@ def test_autotune_keeps_state_and_caches(tmp_path):
         assert e.stats(i).steps == 3
         assert e.schedule(i) in (1, 2)
     lines = open(cache).read().strip().splitlines()
-    assert len(lines) == 2 and all('"schedule": ' in ln for ln in lines)
+    assert len(lines) == 2 and all('"schedule": ' in ln and '"tcs": ' in ln and '"fsets": ' in ln
+                                   for ln in lines)
     e.propagate(5, 0)
     ref.propagate(5, 0)
     for i in range(len(shapes)):
