@@ -58,10 +58,18 @@ def run_batched(args, log):
     for i in range(nch):
         eng.set_params(i, _params(1000 + i, mu=mus[i], nt=Nt))
         eng.init_fields(i, 1.0, 0.1)
+    tuning = None
     if args.schedule:
         eng.set_schedule(args.schedule)
     else:
-        eng.autotune(None)
+        import tempfile
+        fd, path = tempfile.mkstemp(suffix=".jsonl")
+        os.close(fd)
+        eng.autotune(path)
+        with open(path) as f:
+            lines = [json.loads(ln) for ln in f if ln.strip()]
+        os.unlink(path)
+        tuning = lines[0] if lines else None
     log(f"batched: schedule {eng.schedule(0)}, {nch} chains of {X}x{Y}x1x{Nt} FP32, device bytes {eng.device_bytes()/1e9:.2f} GB")
     eng.propagate(args.warmup, 10)
     eng.synchronize()
@@ -81,7 +89,7 @@ def run_batched(args, log):
     sched = eng.schedule(0)
     eng.close()
     return dict(sites=sites, wall=wall, su=sites * args.steps / wall, kt=kt, dt0=st.dt, nonfinite=bad,
-                schedule=sched)
+                schedule=sched, tuning=tuning)
 
 
 def run_plain(args, log):
@@ -361,6 +369,7 @@ def main() -> int:
         "batched_over_plain": round(b["su"] / p["su"], 2),
         "kernel_ms": {k: round(v[0], 3) for k, v in b["kt"].items() if v[1]},
         "roofline": roof,
+        "autotune": b["tuning"],
         "uneven_batch": unev,
         "serving": serv,
         "p50_s": serv["p50_s"] if serv else None,

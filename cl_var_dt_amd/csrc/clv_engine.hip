@@ -1799,7 +1799,8 @@ int json_int(const std::string &line, const char *field, int dflt) {
   return p == std::string::npos ? dflt : std::atoi(line.c_str() + p + k.size());
 }
 
-// cache lines: {"key": "...", "schedule": S, "tcs": T, "fsets": F, "ms": best, ...}
+// cache lines: {"key": "...", "schedule": S, "tcs": T, "fsets": F, "ms": best, ...,
+//               "candidates": [[schedule, tcs, fsets, ms], ...]}
 std::map<std::string, TuneChoice> read_tune_cache(const char *path) {
   std::map<std::string, TuneChoice> out;
   if (!path) return out;
@@ -1960,7 +1961,11 @@ int clv_autotune(clv_ctx *ctx, const char *cache_path) {
       std::ofstream out(cache_path, std::ios::app);
       out << "{\"key\": \"" << key << "\", \"schedule\": " << cand[best].schedule
           << ", \"tcs\": " << b.tcs << ", \"fsets\": " << b.fused_sets << ", \"ms\": " << ms[best]
-          << ", \"ms_two_pass\": " << two << ", \"ms_fused\": " << fused << "}\n";
+          << ", \"ms_two_pass\": " << two << ", \"ms_fused\": " << fused << ", \"candidates\": [";
+      for (size_t q = 0; q < cand.size(); ++q)
+        out << (q ? ", " : "") << "[" << cand[q].schedule << ", " << cand[q].tcs << ", " << cand[q].fsets
+            << ", " << ms[q] << "]";
+      out << "]}\n";
     }
   }
   ctx->profiling = prof;
