@@ -107,3 +107,35 @@ def test_fp32_and_fp64_engine_chains_agree():
         print(f"{name}: fp32 {a.mean():.6f} fp64 {b.mean():.6f} diff {a.mean() - b.mean():+.2e} "
               f"2 sigma {2 * se:.2e} max per-chain |diff| {np.max(abs(a - b)):.2e}")
         assert abs(a.mean() - b.mean()) <= 2 * se + 1e-12
+
+
+@pytest.mark.parametrize("c1", [-0.05, 0.05])
+def test_config1_fp32_vs_fp64_observables(c1):
+    """The plain-method configuration at full length: 2-D 64x64 x Nt=64, beta = 1,
+    mu = 1, c0 = 0.1, q = 0.1, c1 in {-0.05, +0.05}, 5x10^4 adaptive steps,
+    Gaussian start around sqrt(n0 = 1), observables every 10 steps accumulated
+    in FP64 from step 0 (as the reference does).  16 independent chains per
+    precision (seeds 7, 8, ...): the FP32 and FP64 means of density, Sz and
+    energy agree within two standard errors of their difference."""
+    print("case: config 64x64x64 c1 =", c1, flush=True)
+    shape = Shape(64, 64, 1, 64)
+    nch, steps = 16, 50000
+    res = {}
+    for prec in (CLV_FP64, CLV_FP32):
+        eng = _engine([shape] * nch, prec)
+        for i in range(nch):
+            eng.set_params(i, default_params(mu=1.0, G=0.1, c1=c1, Q=0.1, P=0.0, eps=1 / 64,
+                                             DT=1e-3, seed=7 + i))
+            eng.init_fields(i, 1.0, 0.1)
+        eng.propagate(steps, 10)
+        res[prec] = np.array([[eng.observables(i)[0].density, eng.observables(i)[0].Sz,
+                               eng.observables(i)[0].energy] for i in range(nch)])
+        assert all(eng.stats(i).steps == steps and eng.stats(i).nonfinite == 0 for i in range(nch))
+        eng.close()
+    for k, name in enumerate(["density", "Sz", "energy"]):
+        a, b = res[CLV_FP32][:, k], res[CLV_FP64][:, k]
+        se = np.sqrt(a.var(ddof=1) / nch + b.var(ddof=1) / nch)
+        print(f"c1={c1:+.2f} {name}: fp32 {a.mean():.6f} fp64 {b.mean():.6f} "
+              f"diff {a.mean() - b.mean():+.2e} 2 sigma {2 * se:.2e} "
+              f"max per-chain |diff| {np.max(abs(a - b)):.2e}")
+        assert abs(a.mean() - b.mean()) <= 2 * se + 1e-12
