@@ -155,6 +155,35 @@ def test_trajectory_fp64_1000_steps(oracle_lib, plain):
     eng.close()
 
 
+def test_config0_full_length_fp64(oracle_lib):
+    """The plain-method FP64 configuration at full length: 1-D 64 x Nt=32,
+    beta = 1, mu = 1, c0 = 0.1, c1 = -0.05, q = 0.1, Gaussian start around
+    sqrt(n0 = 1), 10^4 adaptive steps, seed 1, host-driven dt per step like the
+    reference: per site within 1e-10 of the oracle, observables to 1e-10."""
+    print("case: config 64x32 fp64 10^4 steps", flush=True)
+    shape = (64, 1, 1, 32)
+    eng, orc, p = _pair(oracle_lib, shape, seed=1, mu=1.0, G=0.1, c1=-0.05, Q=0.1, P=0.0,
+                        eps=1 / 32, DT=1e-3)
+    eng.init_fields(0, 1.0, 0.1)
+    orc.init_fields(1.0, 0.1)
+    eng.propagate_plain(0, 10000, 10)
+    orc.propagate(10000, 10)
+    a, b = eng.get_fields(0)
+    c, d = orc.get_fields()
+    err = max(np.max(abs(a - c)), np.max(abs(b - d)))
+    print("max per-site |delta| after 10^4 steps =", err)
+    assert err <= 1e-10
+    sg, so = eng.stats(0), orc.stats()
+    assert sg.steps == so.steps == 10000
+    assert sg.t_langevin == pytest.approx(so.t_langevin, rel=1e-12)
+    og, _ = eng.observables(0)
+    oo, _ = orc.observables()
+    assert og.count == oo.count == 1000
+    for f in ("n_tot", "density", "Sz", "energy"):
+        assert getattr(og, f) == pytest.approx(getattr(oo, f), rel=1e-10, abs=1e-12), f
+    eng.close()
+
+
 def test_trajectory_fp32_1000_steps(oracle_lib):
     """FP32 fields (FP64 dt, sqrt(dt), reductions) within 1e-4 relative."""
     print("case: fp32 trajectory", flush=True)
