@@ -279,9 +279,10 @@ def test_fused_schedule_bit_identical_to_two_pass(prec):
     reproduces the two-pass schedule bit for bit, observables included."""
     print("case: fused vs two-pass", prec, flush=True)
     from cl_var_dt_amd._abi import SCHED_FUSED, SCHED_TWO_PASS
-    shapes = [(128, 2, 1, 8), (64, 4, 1, 16), (16, 8, 2, 4), (24, 4, 1, 6)]
+    shapes = [(128, 2, 1, 8), (64, 4, 1, 16), (16, 8, 2, 4), (24, 4, 1, 6),
+              (16, 4, 1, 2), (8, 2, 1, 3), (24, 1, 1, 5)]  # short tau: wrap-around and commits
     if prec == CLV_FP64:
-        shapes = [(32, 2, 1, 8), (16, 4, 1, 6)]
+        shapes = [(32, 2, 1, 8), (16, 4, 1, 6), (16, 4, 1, 2), (8, 2, 1, 3)]
     engines = []
     for sched in (SCHED_TWO_PASS, SCHED_FUSED):
         e = _engine(shapes, prec)
@@ -328,6 +329,26 @@ def test_fused_schedule_matches_oracle(oracle_lib):
     rel = max(np.max(abs(a - c)), np.max(abs(b - d))) / scale
     print("fused fp32 rel err", rel)
     assert rel <= 1e-4
+    eng.close()
+
+
+@pytest.mark.parametrize("shape", [(8, 2, 1, 3), (16, 4, 1, 2), (24, 2, 2, 5)])
+def test_fused_fp64_short_tau_matches_oracle(oracle_lib, shape):
+    """Fused FP64 march at Nt = 2, 3, 5 (wrap-around prefetch, the deferred and
+    the tau = 0 partials) against the oracle: 1e-10 per site after 100 steps."""
+    print("case: fused fp64 short tau", shape, flush=True)
+    from cl_var_dt_amd._abi import SCHED_FUSED
+    eng, orc, p = _pair(oracle_lib, shape, seed=5, eps=1 / shape[3])
+    eng.set_schedule(SCHED_FUSED)
+    eng.init_fields(0, 1.0, 0.1)
+    orc.init_fields(1.0, 0.1)
+    eng.propagate(100, 10)
+    orc.propagate(100, 10)
+    a, b = eng.get_fields(0)
+    c, d = orc.get_fields()
+    assert max(np.max(abs(a - c)), np.max(abs(b - d))) <= 1e-10
+    assert eng.stats(0).t_langevin == pytest.approx(orc.stats().t_langevin, rel=1e-12)
+    assert eng.schedule(0) == SCHED_FUSED
     eng.close()
 
 
