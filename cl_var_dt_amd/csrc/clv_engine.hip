@@ -1766,6 +1766,8 @@ int clv_reset_observables(clv_ctx *ctx, int32_t chain) {
   return CLV_OK;
 }
 
+}  // extern "C"
+
 // ---- autotuner: schedule per shape bucket, timed on the live state ----
 namespace {
 
@@ -1912,6 +1914,8 @@ int time_choice(clv_ctx *ctx, Bucket &b, const TuneChoice &c, int nsteps, double
 }
 
 }  // namespace
+
+extern "C" {
 
 int clv_autotune(clv_ctx *ctx, const char *cache_path) {
   if (!ctx) return CLV_ERR_ARG;
