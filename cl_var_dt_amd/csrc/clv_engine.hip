@@ -89,9 +89,7 @@ __global__ __launch_bounds__(256) void k_drift_stats(BucketArgs<T> A) {
         const int tm1 = (t == 0) ? A.Nt - 1 : t - 1;
         km0 = add_kt(km0, k.kK, load1(A.kt_psi + chain * A.chain_stride, (int64_t(tm1) * A.R + r) * X + x, 0));
       }
-      const C2<T> d = drift0_site(k, pm[q], p00, b0, km0);
-      const double dx = double(d.x), dy = double(d.y);
-      v = sqrt(dx * dx + dy * dy);
+      v = drift_abs(drift0_site(k, pm[q], p00, b0, km0));
     }
     double m, s;
     block_max_sum(v, sm.red, tid, nthr, &m, &s);
@@ -237,13 +235,18 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? 5 : 2) void k_update(BucketAr
 // chain for all Nt slices and marches along tau:
 //   1. dt of this step from the partials of the current fields (the same
 //      fixed-order reduction as k_controller; workgroup 0 commits the state),
-//   2. per tau: K(t) and Kbar(t+1) of the old fields and K_0 of the new
-//      psi_0(t-1) in one LDS batch of 9 row sets, drift + Philox noise +
-//      update, and |D_0| of the NEW fields at t-1 -> next step's partials.
+//   2. per tau: K(t) and Kbar(t+1) of the old fields in one LDS batch of 8
+//      row sets, drift + Philox noise + update, and the non-kinetic part of
+//      D_0 of the NEW fields at t (next step's |D_0| partials).  The kinetic
+//      part, K_0 of the new psi_0(t-1), is one more row set per tau; those are
+//      transformed `fbatch` taus at a time (parked in LDS until then), so most
+//      taus transform exactly 8 row sets, which fill the waves of every
+//      transform level.
 // Old fields are read from the current buffer and the new ones written to the
 // other (as the two-pass update does), so the wrap-around neighbours of
 // tau = Nt-1 are still the old tau = 0.  Reads and writes the state once per
-// step; results are bit-identical to the two-pass schedule.
+// step; results are bit-identical to the two-pass schedule for every fbatch
+// and line-set count (the same per-line transforms and drift0_site split).
 template <typename T, int X>
 __global__ __launch_bounds__(256, sizeof(T) == 4 ? CLV_FUSED_WAVES_PER_EU : 2) void k_fused(BucketArgs<T> A) {
   extern __shared__ __align__(16) char smem_raw[];
@@ -256,8 +259,12 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? CLV_FUSED_WAVES_PER_EU : 2) v
     if (tile == 0 && tid == 0) A.st_next[chain] = st;
     return;
   }
-  const int rows = A.rows, Nt = A.Nt;
-  Smem<T, X> sm(smem_raw, 9 * A.fsets * rows);  // fsets sets of 9 row sets
+  const int rows = A.rows, Nt = A.Nt, B = A.fbatch;
+  // line sets of 8 + B row sets: 8 for the old fields' transforms, B slots for
+  // the parked next-step K_0 inputs (used in the set of the batch taus)
+  const int setw = (8 + B) * rows;
+  Smem<T, X> sm(smem_raw, A.fsets * setw);
+  C2<T> *lpark = sm.red_end();  // [B-1][nthr] parked non-kinetic D_0 parts
   using RF = RowFFT<T, X>;
   load_tables<T, X>(sm, A.tw, A.p2, drift_table_scale(P), tid, nthr);
   if (tid < 64) {
@@ -327,24 +334,39 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? CLV_FUSED_WAVES_PER_EU : 2) v
   __syncthreads();  // the LDS lines are rewritten below
   // With two LDS sets the transform lines alternate by tau parity, so a tau's
   // lines can be written while slower waves still read the previous tau's
-  // results (one set: a barrier ends each tau).  Each tau's (max, sum) of
-  // |D_0| is combined across the waves after the next tau's first barrier, in
-  // the order of block_max_sum.
+  // results (one set: a barrier ends each tau).  The batch taus (t % B == B-1)
+  // are odd for B > 1, so the parked slots sit at the end of set (B > 1) * (fsets - 1).
+  // The (max, sum) of |D_0| of each tau are combined across the waves after the
+  // next barrier, in the order of block_max_sum, from slot tau % max(B, 2).
   const int w = tid >> 6, nw = (nthr + 63) >> 6;
-  auto commit_partial = [&](int tt) {  // thread 0, after a barrier
-    const double *rm = sm.red + (tt & 1) * 16;
+  const int rmask = (B < 2 ? 2 : B) - 1;
+  const int park = (B > 1 ? A.fsets - 1 : 0) * setw + 8 * rows;  // first parked slot (B > 1)
+  auto wave_partial = [&](int tau, double v) {
+    const double wm = wave_max(v), ws = wave_sum(v);
+    if ((tid & 63) == 0) {
+      sm.red[(tau & rmask) * 16 + w] = wm;
+      sm.red[(tau & rmask) * 16 + 8 + w] = ws;
+    }
+  };
+  auto commit_partial = [&](int tau) {  // thread 0, after a barrier
+    const double *rm = sm.red + (tau & rmask) * 16;
     double mm = rm[0], ss = rm[8];
     for (int q = 1; q < nw; ++q) {
       mm = fmax(mm, rm[q]);
       ss += rm[8 + q];
     }
-    double *o = A.part_wr + ((int64_t(chain) * Nt + tt) * A.ntiles + tile) * 2;
+    double *o = A.part_wr + ((int64_t(chain) * Nt + tau) * A.ntiles + tile) * 2;
     o[0] = mm;
     o[1] = ss;
   };
+  // the partials of batch tau tb: taus max(1, tb-B+1) .. tb
+  auto commit_batch = [&](int tb) {
+    for (int tau = (tb - B + 1 < 1 ? 1 : tb - B + 1); tau <= tb; ++tau) commit_partial(tau);
+  };
   for (int t = 0; t < Nt; ++t) {
     const int tp1 = (t == Nt - 1) ? 0 : t + 1;
-    const int L0 = (t & (A.fsets - 1)) * 9 * rows;  // first line of this tau's set
+    const int L0 = (t & (A.fsets - 1)) * setw;  // first line of this tau's set
+    const bool batch = t > 0 && (t & (B - 1)) == B - 1;
     C2<T> pp[4], bn[4];
     if (has_site) {
       load4(psi, site_at(tp1), pp);
@@ -355,12 +377,25 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? CLV_FUSED_WAVES_PER_EU : 2) v
         sm.at(L0 + c * rows + j, x) = p0[c];
         sm.at(L0 + (4 + c) * rows + j, x) = bp[c];
       }
-      sm.at(L0 + 8 * rows + j, x) = nprev[0];
+      // K_0 input of tau = t (new psi_0(t-1)); at a batch tau into its own
+      // slot of this set, otherwise parked after this tau's first barrier
+      if (batch) sm.at(L0 + (8 + B - 1) * rows + j, x) = nprev[0];
     }
     __syncthreads();
-    if (tid == 0 && t > 1) commit_partial(t - 1);
-    RF::kinetic(sm.buf + L0 * RF::PAD, sm.tw, sm.mult, 9 * rows, tid, nthr);
-    double v = 0.0;
+    if (tid == 0 && t > 1 && ((t - 1) & (B - 1)) == B - 1) commit_batch(t - 1);
+    if (has_site && !batch && t > 0) sm.at(park + (t & (B - 1)) * rows + j, x) = nprev[0];
+    RF::kinetic(sm.buf + L0 * RF::PAD, sm.tw, sm.mult, (batch ? 8 + B : 8) * rows, tid, nthr);
+    // the parked taus of this batch
+    if (batch) {
+      for (int q = 0; q < B - 1; ++q) {
+        const int tau = t - (B - 1) + q;
+        if (tau < 1) continue;
+        double v = 0.0;
+        if (has_site) v = drift_abs(drift0_add_kin(lpark[q * nthr + tid], sm.at(L0 + (8 + q) * rows + j, x)));
+        wave_partial(tau, v);
+      }
+    }
+    double v_batch = 0.0;
     if (has_site) {
       C2<T> kn[4], kp[4];
 #pragma unroll
@@ -368,7 +403,6 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? CLV_FUSED_WAVES_PER_EU : 2) v
         kn[c] = sm.at(L0 + c * rows + j, x);
         kp[c] = sm.at(L0 + (4 + c) * rows + j, x);
       }
-      const C2<T> k0n = sm.at(L0 + 8 * rows + j, x);
       C2<T> D[4], Db[4];
       drift_site(k, pm, p0, pp, b0, bp, km, kp, D, Db);
       T nz[8] = {};
@@ -388,9 +422,9 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? CLV_FUSED_WAVES_PER_EU : 2) v
       store4(opsi, s0, o);
       store4(opsib, s0, ob);
       if (t > 0) {
-        const C2<T> d0 = drift0_site(k, nprev, o[0], ob, k0n);
-        const double dx = double(d0.x), dy = double(d0.y);
-        v = sqrt(dx * dx + dy * dy);
+        const C2<T> l0 = drift0_local(k, nprev, o[0], ob);
+        if (batch) v_batch = drift_abs(drift0_add_kin(l0, sm.at(L0 + (8 + B - 1) * rows + j, x)));
+        else lpark[(t & (B - 1)) * nthr + tid] = l0;
       }
 #pragma unroll
       for (int c = 0; c < 4; ++c) {
@@ -402,33 +436,44 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? CLV_FUSED_WAVES_PER_EU : 2) v
         nprev[c] = o[c];
       }
     }
-    if (t > 0) {
-      const double wm = wave_max(v), ws = wave_sum(v);
-      if ((tid & 63) == 0) {
-        sm.red[(t & 1) * 16 + w] = wm;
-        sm.red[(t & 1) * 16 + 8 + w] = ws;
-      }
-    }
+    if (batch) wave_partial(t, v_batch);
     if (A.fsets == 1) __syncthreads();  // the one set is rewritten by the next tau
   }
   __syncthreads();
-  if (tid == 0 && Nt > 1) commit_partial(Nt - 1);
+  const int tl = Nt - 1;  // last tau of the march
+  const bool last_batch = tl > 0 && (tl & (B - 1)) == B - 1;
+  if (tid == 0 && last_batch) commit_batch(tl);
+  // K_0 lines still to transform: the taus parked after the last batch
+  // (slots 0 .. np-1; slot 0 unused before the first batch) and tau = 0
+  // (new psi_0(Nt-1)) in the slot after them.  B = 1 uses line 0.
+  const int np = (last_batch || tl == 0) ? 0 : (tl & (B - 1)) + 1;
+  const int pb = (B > 1) ? park : 0;
   // |D_0| of the new tau = 0 (needs the new psi(Nt-1)); this thread wrote
   // the new tau = 0 values of its site at t = 0
   C2<T> n0k[4], nb0k[4];
   if (has_site) {
     load4(opsi, site_at(0), n0k);
     load4(opsib, site_at(0), nb0k);
-    sm.at(j, x) = nprev[0];
+    sm.at(pb + np * rows + j, x) = nprev[0];
   }
   __syncthreads();
-  RF::kinetic(sm.buf, sm.tw, sm.mult, rows, tid, nthr);
-  double v = 0.0;
-  if (has_site) {
-    const C2<T> d0 = drift0_site(k, nprev, n0k[0], nb0k, sm.at(j, x));
-    const double dx = double(d0.x), dy = double(d0.y);
-    v = sqrt(dx * dx + dy * dy);
+  RF::kinetic(sm.buf + pb * RF::PAD, sm.tw, sm.mult, (np + 1) * rows, tid, nthr);
+  if (np > 0) {
+    const int tau0 = tl - (np - 1);  // tau of slot 0
+    for (int q = 0; q < np; ++q) {
+      if (tau0 + q < 1) continue;
+      double v = 0.0;
+      if (has_site) v = drift_abs(drift0_add_kin(lpark[q * nthr + tid], sm.at(pb + q * rows + j, x)));
+      wave_partial(tau0 + q, v);
+    }
+    __syncthreads();
+    if (tid == 0)
+      for (int q = 0; q < np; ++q)
+        if (tau0 + q >= 1) commit_partial(tau0 + q);
+    __syncthreads();  // block_max_sum below rewrites sm.red
   }
+  double v = 0.0;
+  if (has_site) v = drift_abs(drift0_site(k, nprev, n0k[0], nb0k, sm.at(pb + np * rows + j, x)));
   double m, sum;
   block_max_sum(v, sm.red, tid, nthr, &m, &sum);
   if (tid == 0) {
@@ -720,6 +765,7 @@ BucketArgs<T> make_args(const Bucket &b, int chain0, int eval_every, bool fused 
   A.tc = b.tc;
   A.tcs = b.tcs;
   A.fsets = b.fused_sets;
+  A.fbatch = b.fused_batch;
   A.chain0 = chain0;
   A.eval_every = eval_every;
   return A;
@@ -745,8 +791,7 @@ void launch_row_kernels(const Bucket &b, hipStream_t s, int which, int chain0, i
     hipLaunchKernelGGL((k_obs<T, X>), dim3(b.ntiles, b.nchunks, nch), block, sh, s, A);
   } else {
     BucketArgs<T> A = make_args<T>(b, chain0, eval_every, true);
-    const size_t sh = Smem<T, X>::bytes(9 * b.fused_sets * b.rows);
-    hipLaunchKernelGGL((k_fused<T, X>), dim3(b.ntiles, nch), block, sh, s, A);
+    hipLaunchKernelGGL((k_fused<T, X>), dim3(b.ntiles, nch), block, b.fused_lds, s, A);
   }
 }
 
@@ -903,6 +948,14 @@ int split_n2(int X) {
     case 192: return 12;
     default: return 16;
   }
+}
+
+// LDS bytes of the fused kernel with `sets` line sets of (8 + batch) row sets
+// each, plus the parked non-kinetic D_0 parts of batch - 1 taus per thread.
+size_t fused_lds_bytes(const Bucket &b, int sets, int batch) {
+  return size_t(b.esz) * 2 * b.X + size_t(b.esz / 2) * (b.X + (b.X & 1)) +
+         size_t(b.esz) * (8 + batch) * sets * b.rows * row_pad(b.X, split_n2(b.X)) +
+         sizeof(double) * 64 + size_t(b.esz) * (batch - 1) * b.nthr;
 }
 
 // rows per workgroup: the largest divisor of R with X*rows <= max_sites
@@ -1116,12 +1169,18 @@ int setup_bucket(clv_ctx *ctx, Bucket &b) {
   b.host_eval.assign(b.nch, 0);
   b.host_eval_from.assign(b.nch, 0);
   b.host_active.assign(b.nch, 1);
-  const auto fused_lds = [&](int sets) {
-    return size_t(b.esz) * 2 * b.X + size_t(b.esz / 2) * (b.X + (b.X & 1)) +
-           size_t(b.esz) * 9 * sets * b.rows * row_pad(b.X, split_n2(b.X)) + sizeof(double) * 64;
-  };
-  b.fused_sets = b.fused_sets_max = (fused_lds(2) <= 65536) ? 2 : 1;
-  b.fused_lds = (fused_lds(b.fused_sets) <= 65536) ? fused_lds(b.fused_sets) : 0;
+  b.fused_sets = b.fused_sets_max = (fused_lds_bytes(b, 2, 1) <= 65536) ? 2 : 1;
+  b.fused_batch = 1;
+  // Untuned default: one line set with K_0 lines in batches of 4 (else 2)
+  // taus.  Measured on MI355X it is the fastest option or within 3% of it
+  // from 16^3 to 256^2 rows (up to 16% ahead of unbatched).
+  for (int B = 4; B >= 2; B /= 2)
+    if (b.fused_batch == 1 && B <= b.Nt && fused_lds_bytes(b, 1, B) <= 65536) {
+      b.fused_sets = 1;
+      b.fused_batch = B;
+    }
+  b.fused_lds = fused_lds_bytes(b, b.fused_sets, b.fused_batch);
+  if (b.fused_lds > 65536) b.fused_lds = 0;
   b.schedule = auto_schedule(b, b.nch);
   return CLV_OK;
 }
@@ -1786,13 +1845,14 @@ std::string tune_key(const std::string &dev, const Bucket &b, int precision) {
 
 // One tuning choice.  All of them give identical results: the schedules are
 // bit-identical, the reduction tile (tcs tau slices per drift-statistics
-// workgroup) does not change the per-(tau, tile) partials, and the fused
-// kernel's LDS line sets only change its barriers.  The row tile and the
+// workgroup) does not change the per-(tau, tile) partials, the fused
+// kernel's LDS line sets only change its barriers, and its K_0 batch only
+// groups the same per-line transforms differently.  The row tile and the
 // radix split are fixed per shape: they set the partial-sum granularity and
 // the transform's rounding, so tuning them would make results depend on a
 // timing measurement.
 struct TuneChoice {
-  int schedule = 1, tcs = 1, fsets = 1;
+  int schedule = 1, tcs = 1, fsets = 1, fbatch = 1;
 };
 
 int json_int(const std::string &line, const char *field, int dflt) {
@@ -1801,8 +1861,8 @@ int json_int(const std::string &line, const char *field, int dflt) {
   return p == std::string::npos ? dflt : std::atoi(line.c_str() + p + k.size());
 }
 
-// cache lines: {"key": "...", "schedule": S, "tcs": T, "fsets": F, "ms": best, ...,
-//               "candidates": [[schedule, tcs, fsets, ms], ...]}
+// cache lines: {"key": "...", "schedule": S, "tcs": T, "fsets": F, "fbatch": B, "ms": best, ...,
+//               "candidates": [[schedule, tcs, fsets, fbatch, ms], ...]}
 std::map<std::string, TuneChoice> read_tune_cache(const char *path) {
   std::map<std::string, TuneChoice> out;
   if (!path) return out;
@@ -1816,6 +1876,7 @@ std::map<std::string, TuneChoice> read_tune_cache(const char *path) {
     c.schedule = json_int(line, "schedule", 1);
     c.tcs = json_int(line, "tcs", 0);
     c.fsets = json_int(line, "fsets", 0);
+    c.fbatch = json_int(line, "fbatch", 1);
     out[line.substr(k0 + 8, k1 - k0 - 8)] = c;
   }
   return out;
@@ -1867,30 +1928,37 @@ int restore_snapshot(clv_ctx *ctx, Bucket &b, Snapshot &sn, bool release) {
 }
 
 // Valid choices for bucket b (reduction tiles that divide Nt and fit LDS;
-// fused line sets that fit 64 KB).
-std::vector<TuneChoice> tune_choices(const Bucket &b, int fused_sets_max) {
+// fused line sets and K_0 batches that fit 64 KB, batches of at most Nt).
+std::vector<TuneChoice> tune_choices(const Bucket &b) {
   std::vector<TuneChoice> out;
   const size_t line = size_t(b.esz) * row_pad(b.X, split_n2(b.X));
   for (int q = 1; q <= 4; ++q)
     if (b.Nt % q == 0 && q * b.rows * b.X <= 1024 && line * q * b.rows <= 48 * 1024)
-      out.push_back({1, q, 1});
+      out.push_back({1, q, 1, 1});
   if (b.fused_lds > 0 && !b.kt_on)
-    for (int f = 1; f <= fused_sets_max; ++f) out.push_back({2, b.tcs, f});
+    for (int f = 1; f <= b.fused_sets_max; ++f)
+      for (int B = 1; B <= 4; B *= 2)
+        if ((B == 1 || B <= b.Nt) && fused_lds_bytes(b, f, B) <= 65536) out.push_back({2, b.tcs, f, B});
   return out;
 }
 
-bool choice_valid(const Bucket &b, const TuneChoice &c, int fused_sets_max) {
-  for (const TuneChoice &v : tune_choices(b, fused_sets_max))
+bool choice_valid(const Bucket &b, const TuneChoice &c) {
+  for (const TuneChoice &v : tune_choices(b))
     if (v.schedule == c.schedule && (c.schedule == 2 || v.tcs == c.tcs) &&
-        (c.schedule == 1 || v.fsets == c.fsets))
+        (c.schedule == 1 || (v.fsets == c.fsets && v.fbatch == c.fbatch)))
       return true;
   return false;
 }
 
 void apply_choice(Bucket &b, const TuneChoice &c) {
   b.schedule = c.schedule;
-  if (c.schedule == 1) b.tcs = c.tcs;
-  else b.fused_sets = c.fsets;
+  if (c.schedule == 1) {
+    b.tcs = c.tcs;
+  } else {
+    b.fused_sets = c.fsets;
+    b.fused_batch = c.fbatch;
+    b.fused_lds = fused_lds_bytes(b, c.fsets, c.fbatch);
+  }
   b.primed = false;
 }
 
@@ -1928,13 +1996,12 @@ int clv_autotune(clv_ctx *ctx, const char *cache_path) {
   int rc = CLV_OK;
   for (Bucket &b : ctx->buckets) {
     const std::string key = tune_key(dev, b, ctx->precision) + "|" + std::to_string(b.nch) + "ch";
-    const int fsmax = b.fused_sets_max;
     auto hit = cache.find(key);
-    if (hit != cache.end() && choice_valid(b, hit->second, fsmax)) {
+    if (hit != cache.end() && choice_valid(b, hit->second)) {
       apply_choice(b, hit->second);
       continue;
     }
-    const std::vector<TuneChoice> cand = tune_choices(b, fsmax);
+    const std::vector<TuneChoice> cand = tune_choices(b);
     if (cand.size() <= 1) {
       if (!cand.empty()) apply_choice(b, cand[0]);
       continue;
@@ -1964,11 +2031,12 @@ int clv_autotune(clv_ctx *ctx, const char *cache_path) {
       }
       std::ofstream out(cache_path, std::ios::app);
       out << "{\"key\": \"" << key << "\", \"schedule\": " << cand[best].schedule
-          << ", \"tcs\": " << b.tcs << ", \"fsets\": " << b.fused_sets << ", \"ms\": " << ms[best]
+          << ", \"tcs\": " << b.tcs << ", \"fsets\": " << b.fused_sets << ", \"fbatch\": " << b.fused_batch
+          << ", \"ms\": " << ms[best]
           << ", \"ms_two_pass\": " << two << ", \"ms_fused\": " << fused << ", \"candidates\": [";
       for (size_t q = 0; q < cand.size(); ++q)
         out << (q ? ", " : "") << "[" << cand[q].schedule << ", " << cand[q].tcs << ", " << cand[q].fsets
-            << ", " << ms[q] << "]";
+            << ", " << cand[q].fbatch << ", " << ms[q] << "]";
       out << "]}\n";
     }
   }
@@ -1989,6 +2057,28 @@ int clv_set_schedule(clv_ctx *ctx, int32_t schedule) {
       return fail(ctx, CLV_ERR_ARG, "clv_set_schedule: chains with kin_dims = 3 run the two-pass schedule");
   }
   for (Bucket &b : ctx->buckets) b.schedule = (schedule == 0) ? auto_schedule(b, b.nch) : schedule;
+  drop_graphs(ctx);
+  return CLV_OK;
+}
+
+int clv_set_fused_options(clv_ctx *ctx, int32_t line_sets, int32_t batch) {
+  if (!ctx) return CLV_ERR_ARG;
+  if (line_sets < 0 || line_sets > 2 || !(batch == 0 || batch == 1 || batch == 2 || batch == 4))
+    return fail(ctx, CLV_ERR_ARG, "clv_set_fused_options: line_sets must be 0, 1 or 2 and batch 0, 1, 2 or 4");
+  for (const Bucket &b : ctx->buckets) {
+    if (b.fused_lds == 0) continue;  // no fused kernel for this shape
+    const int f = line_sets ? line_sets : b.fused_sets, B = batch ? batch : b.fused_batch;
+    if (f > b.fused_sets_max || fused_lds_bytes(b, f, B) > 65536)
+      return fail(ctx, CLV_ERR_ARG, "clv_set_fused_options: " + std::to_string(f) + " line sets with batch " +
+                                        std::to_string(B) + " do not fit LDS for shape " + std::to_string(b.X) +
+                                        "x" + std::to_string(b.R) + "x" + std::to_string(b.Nt));
+  }
+  for (Bucket &b : ctx->buckets) {
+    if (b.fused_lds == 0) continue;
+    if (line_sets) b.fused_sets = line_sets;
+    if (batch) b.fused_batch = batch;
+    b.fused_lds = fused_lds_bytes(b, b.fused_sets, b.fused_batch);
+  }
   drop_graphs(ctx);
   return CLV_OK;
 }

@@ -44,9 +44,10 @@ struct Bucket {
   int epoch = 0;         // slot (epoch & 1) of d_st / d_part is current
   bool primed = false;   // current partials describe the current fields
   size_t part_slot = 0;  // doubles per partials slot
-  size_t fused_lds = 0;  // LDS bytes of the fused kernel (0: not available)
+  size_t fused_lds = 0;  // LDS bytes of the fused kernel as configured (0: not available)
   int fused_sets = 1;    // LDS line sets of the fused kernel (2 when they fit)
   int fused_sets_max = 1;
+  int fused_batch = 1;   // tau slices per batch of next-step K_0 lines (1, 2, 4)
   // kin_dims = 3: transverse kinetic term kt (allocated when a chain asks for it)
   struct KtAxis {
     int L = 1, bmul = 0, rstride = 0, nbase = 0, W = 1;

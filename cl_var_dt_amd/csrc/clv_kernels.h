@@ -34,7 +34,8 @@ struct BucketArgs {
   int R, Nt, rows, ntiles, nchunks, tc;
   int tcs;                      // tau slices per drift-stats workgroup
   int fsets;                    // fused kernel: LDS line sets (2: alternate by tau parity)
-  int chain0;                   // first chain of the launch
+  int fbatch;                   // fused kernel: tau slices per batch of next-step K_0 lines (1, 2, 4)
+  int chain0;                  // first chain of the launch
   int eval_every;               // obs kernels: evaluate chains with cur_step % eval_every == 0;
                                 // < 0: each chain's own ChainParams::eval_every; 0: always
 };
@@ -205,14 +206,37 @@ __device__ __forceinline__ void drift_site(const Coef<T> &k, const C2<T> pm[4], 
   }
 }
 
-// Component 0 of the psi drift only (what the max/mean reduction reads, Q2).
+// Component 0 of the psi drift only (what the max/mean reduction reads, Q2),
+// in two parts: everything but the kinetic term, and then + K'_0(t-1) as the
+// last operation, so the fused march can form the first part at one tau and
+// add the kinetic term after a later, batched transform with the same bits.
+template <typename T>
+__device__ __forceinline__ C2<T> drift0_local(const Coef<T> &k, const C2<T> pm[4], C2<T> p00,
+                                              const C2<T> b0[4]) {
+  const C2<T> N = cdot4(pm, b0), det = cdet(pm);
+  const C2<T> B = {k.kB * det.x, k.kB * det.y};
+  const C2<T> c = {k.cs[0] - k.kA * N.x, -(k.kA * N.y)};
+  const C2<T> f1 = pm[0], cof = b0[3];
+  return {k.af * (f1.x - p00.x) + (c.x * f1.x - c.y * f1.y) + (B.x * cof.x - B.y * cof.y),
+          k.af * (f1.y - p00.y) + (c.x * f1.y + c.y * f1.x) + (B.x * cof.y + B.y * cof.x)};
+}
+
+template <typename T>
+__device__ __forceinline__ C2<T> drift0_add_kin(C2<T> local, C2<T> km0) {
+  return {local.x + km0.x, local.y + km0.y};
+}
+
 template <typename T>
 __device__ __forceinline__ C2<T> drift0_site(const Coef<T> &k, const C2<T> pm[4], C2<T> p00,
                                              const C2<T> b0[4], C2<T> km0) {
-  const C2<T> N = cdot4(pm, b0), det = cdet(pm);
-  const C2<T> B = {k.kB * det.x, k.kB * det.y};
-  return drift_term(k.af, pm[0], p00, km0, C2<T>{k.cs[0] - k.kA * N.x, -(k.kA * N.y)}, pm[0], B,
-                    b0[3]);
+  return drift0_add_kin(drift0_local(k, pm, p00, b0), km0);
+}
+
+// |D_0| in FP64 (the reduction's input)
+template <typename T>
+__device__ __forceinline__ double drift_abs(C2<T> d) {
+  const double dx = double(d.x), dy = double(d.y);
+  return sqrt(dx * dx + dy * dy);
 }
 
 // Shared-memory carve-up of the row-transform kernels: twiddles, the

@@ -142,6 +142,10 @@ class Engine:
     def schedule(self, chain: int = 0) -> int:
         return int(lib().clv_get_schedule(self._h, chain))
 
+    def set_fused_options(self, line_sets: int = 0, batch: int = 0) -> None:
+        """Fused schedule: LDS line sets (1, 2) and K_0 batch (1, 2, 4); 0 keeps."""
+        self._chk(lib().clv_set_fused_options(self._h, line_sets, batch))
+
     def set_graphs(self, on: bool) -> None:
         self._chk(lib().clv_set_graphs(self._h, 1 if on else 0))
 

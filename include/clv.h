@@ -188,6 +188,13 @@ int clv_autotune(clv_ctx *ctx, const char *cache_path);
 int clv_set_schedule(clv_ctx *ctx, int32_t schedule);
 /* The schedule the chain's bucket runs (1 or 2). */
 int clv_get_schedule(const clv_ctx *ctx, int32_t chain);
+/* Options of the fused schedule for every bucket that has it (project
+ * extension, no counterpart in the reference).  line_sets: LDS line sets,
+ * 1 or 2 (two alternate by tau parity); batch: tau slices whose next-step
+ * K_0 row transforms run together, 1, 2 or 4.  0 keeps the current value.
+ * Results do not depend on either; a combination that does not fit 64 KB of
+ * LDS for some shape is rejected.  clv_autotune times them. */
+int clv_set_fused_options(clv_ctx *ctx, int32_t line_sets, int32_t batch);
 /* clv_propagate replays a captured HIP graph of two steps of every bucket
  * (on by default); off runs every step eagerly.  Results are identical. */
 int clv_set_graphs(clv_ctx *ctx, int32_t on);

@@ -312,6 +312,70 @@ def test_fused_schedule_bit_identical_to_two_pass(prec):
         e.close()
 
 
+@pytest.mark.parametrize("prec", [CLV_FP32, CLV_FP64])
+def test_fused_options_bit_identical_to_two_pass(prec):
+    """Every fused option (LDS line sets 1/2, next-step K_0 lines transformed
+    1/2/4 taus at a time) reproduces the two-pass schedule bit for bit: Nt from
+    2 to 9 and 32 cover batches that end on, before and after the last tau,
+    the parked taus flushed with tau = 0, and a first batch without tau = 0."""
+    print("case: fused options", prec, flush=True)
+    from cl_var_dt_amd import ClvError
+    from cl_var_dt_amd._abi import SCHED_FUSED, SCHED_TWO_PASS
+    shapes = [(128, 2, 1, 32), (128, 2, 1, 5), (64, 4, 1, 7), (16, 8, 2, 9), (24, 4, 1, 6),
+              (16, 4, 1, 2), (8, 2, 1, 3), (32, 2, 1, 4), (128, 128, 1, 8)]
+    if prec == CLV_FP64:
+        shapes = [(32, 2, 1, 8), (16, 4, 1, 6), (16, 4, 1, 5), (8, 2, 1, 3), (8, 4, 1, 7)]
+
+    def run(sched, opts=None):
+        e = _engine(shapes, prec)
+        for i in range(len(shapes)):
+            e.set_params(i, _params(seed=60 + i, mu=0.6 + 0.1 * i))
+            e.init_fields(i, 1.0, 0.1)
+        e.set_schedule(sched)
+        if opts is not None:
+            e.set_fused_options(*opts)
+        e.propagate(7, 3)
+        e.propagate(5, 3)
+        return e
+
+    ref = run(SCHED_TWO_PASS)
+    tried = 0
+    for opts in [(1, 1), (2, 1), (1, 2), (2, 2), (1, 4), (2, 4)]:
+        print("  options", opts, flush=True)
+        try:
+            e = run(SCHED_FUSED, opts)
+        except ClvError as exc:  # does not fit LDS for some shape at this precision
+            print("  skipped:", exc, flush=True)
+            continue
+        tried += 1
+        assert e.schedule(0) == SCHED_FUSED
+        for i in range(len(shapes)):
+            fa, ga = ref.get_fields(i)
+            fb, gb = e.get_fields(i)
+            assert np.array_equal(fa, fb) and np.array_equal(ga, gb), (opts, shapes[i])
+            sa, sb = ref.stats(i), e.stats(i)
+            assert (sa.dt, sa.max_drift, sa.mean_drift, sa.t_langevin) == \
+                   (sb.dt, sb.max_drift, sb.mean_drift, sb.t_langevin), (opts, shapes[i])
+            oa, _ = ref.observables(i)
+            ob, _ = e.observables(i)
+            assert oa.n_tot == ob.n_tot and oa.energy == ob.energy, (opts, shapes[i])
+        e.close()
+    ref.close()
+    assert tried >= (4 if prec == CLV_FP32 else 1)
+
+
+def test_fused_options_rejected_when_too_big():
+    print("case: fused options errors", flush=True)
+    from cl_var_dt_amd import ClvError
+    e = _engine([(16, 4, 1, 4)])
+    with pytest.raises(ClvError, match="batch"):
+        e.set_fused_options(1, 3)
+    with pytest.raises(ClvError, match="line_sets"):
+        e.set_fused_options(3, 1)
+    e.set_fused_options(0, 2)
+    e.close()
+
+
 def test_fused_schedule_matches_oracle(oracle_lib):
     """Fused FP32 batched step vs the FP64 oracle (1e-4 relative after 200 steps)."""
     print("case: fused vs oracle", flush=True)
