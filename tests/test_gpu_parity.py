@@ -364,6 +364,47 @@ def test_fused_options_bit_identical_to_two_pass(prec):
     assert tried >= (4 if prec == CLV_FP32 else 1)
 
 
+@pytest.mark.parametrize("prec", [CLV_FP32, CLV_FP64])
+def test_observables_identical_across_schedules_and_graphs(prec):
+    """Observables (n_c, energy, count, F_a(x)) and fields are the same bit for
+    bit for two-pass eager steps, fused eager steps and fused graph-replayed
+    steps, over calls of 11, 6, 1 and 4 steps with evaluations every 3, 2, 1
+    and 5 steps, at Nt with one and several observable tau-chunks (9 = 3
+    chunks of 3, 32 = 4 of 8)."""
+    print("case: observables across schedules", prec, flush=True)
+    from cl_var_dt_amd._abi import SCHED_FUSED, SCHED_TWO_PASS
+    shapes = [(128, 2, 1, 32), (64, 4, 1, 9), (16, 8, 2, 6), (32, 2, 1, 3)]
+    if prec == CLV_FP64:
+        shapes = [(32, 2, 1, 8), (16, 4, 1, 9), (8, 4, 1, 3)]
+
+    def run(sched, graphs):
+        e = _engine(shapes, prec)
+        for i in range(len(shapes)):
+            e.set_params(i, _params(seed=90 + i, mu=0.5 + 0.2 * i))
+            e.init_fields(i, 1.0, 0.1)
+        e.set_schedule(sched)
+        e.set_graphs(graphs)
+        for n, ev in [(11, 3), (6, 2), (1, 1), (4, 5)]:
+            e.propagate(n, ev)
+        return e
+
+    ref = run(SCHED_TWO_PASS, False)
+    for label, e in [("fused eager", run(SCHED_FUSED, False)), ("fused graphs", run(SCHED_FUSED, True))]:
+        for i in range(len(shapes)):
+            fa, ga = ref.get_fields(i)
+            fb, gb = e.get_fields(i)
+            assert np.array_equal(fa, fb) and np.array_equal(ga, gb), (label, shapes[i])
+            oa, Fa = ref.observables(i)
+            ob, Fb = e.observables(i)
+            assert oa.count == ob.count and oa.count > 0, (label, shapes[i])
+            assert (oa.n_tot, oa.energy, oa.n_comp[0], oa.n_comp[3]) == \
+                (ob.n_tot, ob.energy, ob.n_comp[0], ob.n_comp[3]), (label, shapes[i])
+            for a, b in zip(Fa, Fb):
+                assert np.array_equal(a, b), (label, shapes[i])
+        e.close()
+    ref.close()
+
+
 def test_fused_options_rejected_when_too_big():
     print("case: fused options errors", flush=True)
     from cl_var_dt_amd import ClvError
