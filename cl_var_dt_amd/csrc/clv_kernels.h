@@ -232,11 +232,15 @@ __device__ __forceinline__ C2<T> drift0_site(const Coef<T> &k, const C2<T> pm[4]
   return drift0_add_kin(drift0_local(k, pm, p00, b0), km0);
 }
 
-// |D_0| in FP64 (the reduction's input)
+// |D_0| in FP64 (the reduction's input).  With FP32 fields the square sum
+// is exact in FP64 and the hardware square root (v_sqrt_f64, not correctly
+// rounded but far inside FP32's precision) replaces the correctly rounded
+// sequence; FP64 fields keep the latter.
 template <typename T>
 __device__ __forceinline__ double drift_abs(C2<T> d) {
   const double dx = double(d.x), dy = double(d.y);
-  return sqrt(dx * dx + dy * dy);
+  if constexpr (sizeof(T) == 4) return __builtin_amdgcn_sqrt(dx * dx + dy * dy);
+  else return sqrt(dx * dx + dy * dy);
 }
 
 // Shared-memory carve-up of the row-transform kernels: twiddles, the
