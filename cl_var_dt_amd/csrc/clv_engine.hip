@@ -38,9 +38,9 @@ using namespace clvi;
 
 // ============================================================== kernels ====
 
-// D_0 at every site of one tau-slice of `rows` rows; writes the tile's
+// D_0 at every site of TC tau-slices of `rows` rows; writes each slice's tile
 // (max |D_0|, sum |D_0|) -- compute_max_and_mean_drift, Propagator_without_cutoff.cu:231-265,
-// in FP64 and without atomics.
+// in FP64 and without atomics (tile_max_sum, one wave per slice).
 template <typename T, int X>
 __global__ __launch_bounds__(256) void k_drift_stats(BucketArgs<T> A) {
   extern __shared__ __align__(16) char smem_raw[];
@@ -53,6 +53,7 @@ __global__ __launch_bounds__(256) void k_drift_stats(BucketArgs<T> A) {
   const bool has_site = j < A.rows;
   const int rows = A.rows, TC = A.tcs;
   Smem<T, X> sm(smem_raw, TC * rows);
+  double *vb = reinterpret_cast<double *>(sm.red_end());  // [TC][nthr] |D_0|
   load_tables<T, X>(sm, A.tw, A.p2, drift_table_scale(P), tid, nthr);
   const int par = A.st[chain].parity;  // current buffer
   const C2<T> *psi = (par ? A.buf1_psi : A.buf0_psi) + chain * A.chain_stride;
@@ -91,14 +92,18 @@ __global__ __launch_bounds__(256) void k_drift_stats(BucketArgs<T> A) {
       }
       v = drift_abs(drift0_site(k, pm[q], p00, b0, km0));
     }
+    vb[q * nthr + tid] = v;
+  }
+  __syncthreads();
+  const int w = tid >> 6, nw = nthr >> 6, lane = tid & 63;
+  for (int q = w; q < TC; q += nw) {
     double m, s;
-    block_max_sum(v, sm.red, tid, nthr, &m, &s);
-    if (tid == 0) {
-      double *o = A.part_wr + ((int64_t(chain) * A.Nt + t) * A.ntiles + tile) * 2;
+    tile_max_sum(vb + q * nthr, nthr, lane, &m, &s);
+    if (lane == 0) {
+      double *o = A.part_wr + ((int64_t(chain) * A.Nt + t0 + q) * A.ntiles + tile) * 2;
       o[0] = m;
       o[1] = s;
     }
-    __syncthreads();  // sm.red is reused by the next slice
   }
 }
 
@@ -265,6 +270,7 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? CLV_FUSED_WAVES_PER_EU : 2) v
   const int setw = (8 + B) * rows;
   Smem<T, X> sm(smem_raw, A.fsets * setw);
   C2<T> *lpark = sm.red_end();  // [B-1][nthr] parked non-kinetic D_0 parts
+  double *vb = reinterpret_cast<double *>(lpark + (B - 1) * nthr);  // [max(B,2)][nthr] |D_0|
   using RF = RowFFT<T, X>;
   load_tables<T, X>(sm, A.tw, A.p2, drift_table_scale(P), tid, nthr);
   if (tid < 64) {
@@ -336,32 +342,28 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? CLV_FUSED_WAVES_PER_EU : 2) v
   // lines can be written while slower waves still read the previous tau's
   // results (one set: a barrier ends each tau).  The batch taus (t % B == B-1)
   // are odd for B > 1, so the parked slots sit at the end of set (B > 1) * (fsets - 1).
-  // The (max, sum) of |D_0| of each tau are combined across the waves after the
-  // next barrier, in the order of block_max_sum, from slot tau % max(B, 2).
-  const int w = tid >> 6, nw = (nthr + 63) >> 6;
+  // Each tau's |D_0| go to LDS slot tau % max(B, 2); after the next barrier
+  // one wave per tau reduces them (tile_max_sum) and writes the partial, the
+  // B taus of a batch spread over the waves.
+  const int w = tid >> 6, nw = nthr >> 6, lane = tid & 63;
   const int rmask = (B < 2 ? 2 : B) - 1;
   const int park = (B > 1 ? A.fsets - 1 : 0) * setw + 8 * rows;  // first parked slot (B > 1)
-  auto wave_partial = [&](int tau, double v) {
-    const double wm = wave_max(v), ws = wave_sum(v);
-    if ((tid & 63) == 0) {
-      sm.red[(tau & rmask) * 16 + w] = wm;
-      sm.red[(tau & rmask) * 16 + 8 + w] = ws;
+  auto put_v = [&](int tau, double v) { vb[(tau & rmask) * nthr + tid] = v; };
+  auto partial = [&](int tau, int slot) {  // one whole wave, after a barrier
+    double m, s;
+    tile_max_sum(vb + slot * nthr, nthr, lane, &m, &s);
+    if (lane == 0) {
+      double *o = A.part_wr + ((int64_t(chain) * Nt + tau) * A.ntiles + tile) * 2;
+      o[0] = m;
+      o[1] = s;
     }
   };
-  auto commit_partial = [&](int tau) {  // thread 0, after a barrier
-    const double *rm = sm.red + (tau & rmask) * 16;
-    double mm = rm[0], ss = rm[8];
-    for (int q = 1; q < nw; ++q) {
-      mm = fmax(mm, rm[q]);
-      ss += rm[8 + q];
+  // the partials of batch tau tb: taus max(1, tb-B+1) .. tb, q-th by wave q % nw
+  auto reduce_batch = [&](int tb) {
+    for (int q = w; q < B; q += nw) {
+      const int tau = tb - B + 1 + q;
+      if (tau >= 1) partial(tau, tau & rmask);
     }
-    double *o = A.part_wr + ((int64_t(chain) * Nt + tau) * A.ntiles + tile) * 2;
-    o[0] = mm;
-    o[1] = ss;
-  };
-  // the partials of batch tau tb: taus max(1, tb-B+1) .. tb
-  auto commit_batch = [&](int tb) {
-    for (int tau = (tb - B + 1 < 1 ? 1 : tb - B + 1); tau <= tb; ++tau) commit_partial(tau);
   };
   for (int t = 0; t < Nt; ++t) {
     const int tp1 = (t == Nt - 1) ? 0 : t + 1;
@@ -382,7 +384,7 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? CLV_FUSED_WAVES_PER_EU : 2) v
       if (batch) sm.at(L0 + (8 + B - 1) * rows + j, x) = nprev[0];
     }
     __syncthreads();
-    if (tid == 0 && t > 1 && ((t - 1) & (B - 1)) == B - 1) commit_batch(t - 1);
+    if (t > 1 && ((t - 1) & (B - 1)) == B - 1) reduce_batch(t - 1);
     if (has_site && !batch && t > 0) sm.at(park + (t & (B - 1)) * rows + j, x) = nprev[0];
     RF::kinetic(sm.buf + L0 * RF::PAD, sm.tw, sm.mult, (batch ? 8 + B : 8) * rows, tid, nthr);
     // the parked taus of this batch
@@ -392,7 +394,7 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? CLV_FUSED_WAVES_PER_EU : 2) v
         if (tau < 1) continue;
         double v = 0.0;
         if (has_site) v = drift_abs(drift0_add_kin(lpark[q * nthr + tid], sm.at(L0 + (8 + q) * rows + j, x)));
-        wave_partial(tau, v);
+        put_v(tau, v);
       }
     }
     double v_batch = 0.0;
@@ -436,13 +438,13 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? CLV_FUSED_WAVES_PER_EU : 2) v
         nprev[c] = o[c];
       }
     }
-    if (batch) wave_partial(t, v_batch);
+    if (batch) put_v(t, v_batch);
     if (A.fsets == 1) __syncthreads();  // the one set is rewritten by the next tau
   }
   __syncthreads();
   const int tl = Nt - 1;  // last tau of the march
   const bool last_batch = tl > 0 && (tl & (B - 1)) == B - 1;
-  if (tid == 0 && last_batch) commit_batch(tl);
+  if (last_batch) reduce_batch(tl);
   // K_0 lines still to transform: the taus parked after the last batch
   // (slots 0 .. np-1; slot 0 unused before the first batch) and tau = 0
   // (new psi_0(Nt-1)) in the slot after them.  B = 1 uses line 0.
@@ -458,28 +460,22 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? CLV_FUSED_WAVES_PER_EU : 2) v
   }
   __syncthreads();
   RF::kinetic(sm.buf + pb * RF::PAD, sm.tw, sm.mult, (np + 1) * rows, tid, nthr);
-  if (np > 0) {
-    const int tau0 = tl - (np - 1);  // tau of slot 0
-    for (int q = 0; q < np; ++q) {
-      if (tau0 + q < 1) continue;
-      double v = 0.0;
-      if (has_site) v = drift_abs(drift0_add_kin(lpark[q * nthr + tid], sm.at(pb + q * rows + j, x)));
-      wave_partial(tau0 + q, v);
-    }
-    __syncthreads();
-    if (tid == 0)
-      for (int q = 0; q < np; ++q)
-        if (tau0 + q >= 1) commit_partial(tau0 + q);
-    __syncthreads();  // block_max_sum below rewrites sm.red
+  // the pending taus in slots 0 .. np-1 (the slot of tau0 + q is q), tau = 0
+  // in slot np; items i = 0 .. np, item i reduced by wave i % nw
+  const int tau0 = tl - (np - 1);
+  for (int q = 0; q < np; ++q) {
+    if (tau0 + q < 1) continue;
+    double v = 0.0;
+    if (has_site) v = drift_abs(drift0_add_kin(lpark[q * nthr + tid], sm.at(pb + q * rows + j, x)));
+    vb[q * nthr + tid] = v;
   }
-  double v = 0.0;
-  if (has_site) v = drift_abs(drift0_site(k, nprev, n0k[0], nb0k, sm.at(pb + np * rows + j, x)));
-  double m, sum;
-  block_max_sum(v, sm.red, tid, nthr, &m, &sum);
-  if (tid == 0) {
-    double *o = A.part_wr + (int64_t(chain) * Nt * A.ntiles + tile) * 2;
-    o[0] = m;
-    o[1] = sum;
+  double v0 = 0.0;
+  if (has_site) v0 = drift_abs(drift0_site(k, nprev, n0k[0], nb0k, sm.at(pb + np * rows + j, x)));
+  vb[np * nthr + tid] = v0;
+  __syncthreads();
+  for (int i = w; i <= np; i += nw) {
+    if (i < np && tau0 + i < 1) continue;
+    partial(i < np ? tau0 + i : 0, i);
   }
 }
 
@@ -779,7 +775,7 @@ void launch_row_kernels(const Bucket &b, hipStream_t s, int which, int chain0, i
   const dim3 block(b.nthr);
   if (which == 0) {
     BucketArgs<T> A = make_args<T>(b, chain0, eval_every);
-    const size_t sh = Smem<T, X>::bytes(b.tcs * b.rows);
+    const size_t sh = Smem<T, X>::bytes(b.tcs * b.rows) + sizeof(double) * b.tcs * b.nthr;
     hipLaunchKernelGGL((k_drift_stats<T, X>), dim3(b.ntiles, b.Nt / b.tcs, nch), block, sh, s, A);
   } else if (which == 1) {
     BucketArgs<T> A = make_args<T>(b, chain0, eval_every);
@@ -951,11 +947,13 @@ int split_n2(int X) {
 }
 
 // LDS bytes of the fused kernel with `sets` line sets of (8 + batch) row sets
-// each, plus the parked non-kinetic D_0 parts of batch - 1 taus per thread.
+// each, plus the parked non-kinetic D_0 parts of batch - 1 taus per thread
+// and |D_0| of max(batch, 2) taus per thread for the partials.
 size_t fused_lds_bytes(const Bucket &b, int sets, int batch) {
   return size_t(b.esz) * 2 * b.X + size_t(b.esz / 2) * (b.X + (b.X & 1)) +
          size_t(b.esz) * (8 + batch) * sets * b.rows * row_pad(b.X, split_n2(b.X)) +
-         sizeof(double) * 64 + size_t(b.esz) * (batch - 1) * b.nthr;
+         sizeof(double) * 64 + size_t(b.esz) * (batch - 1) * b.nthr +
+         sizeof(double) * (batch < 2 ? 2 : batch) * b.nthr;
 }
 
 // rows per workgroup: the largest divisor of R with X*rows <= max_sites

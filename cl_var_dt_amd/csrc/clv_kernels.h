@@ -307,24 +307,20 @@ __device__ __forceinline__ void reduce_partials(const double *pp, int n, int lan
   *s = wave_sum(ss);
 }
 
-// Workgroup (max, sum) of v in a fixed order; valid in thread 0.  Uses red[0..63].
-__device__ __forceinline__ void block_max_sum(double v, double *red, int tid, int nthr, double *m,
-                                              double *s) {
-  double mm = wave_max(v), ss = wave_sum(v);
-  const int w = tid >> 6, nw = (nthr + 63) >> 6;
-  if ((tid & 63) == 0) {
-    red[w] = mm;
-    red[32 + w] = ss;
+// (max, sum) of one tile's nthr values vb[0..nthr) in a fixed order, by one
+// wave: lane l folds vb[l], vb[l+64], ... in order, then the wave trees.
+// Valid in lane 0.  The drift-statistics partials of both schedules come from
+// here, so only one wave per tile and tau runs the cross-lane trees.
+__device__ __forceinline__ void tile_max_sum(const double *vb, int nthr, int lane, double *m,
+                                             double *s) {
+  double mm = vb[lane], ss = vb[lane];
+  for (int i = lane + 64; i < nthr; i += 64) {
+    const double v = vb[i];
+    mm = fmax(mm, v);
+    ss += v;
   }
-  __syncthreads();
-  if (tid == 0) {
-    for (int q = 1; q < nw; ++q) {
-      mm = fmax(mm, red[q]);
-      ss += red[32 + q];
-    }
-  }
-  *m = mm;
-  *s = ss;
+  *m = wave_max(mm);
+  *s = wave_sum(ss);
 }
 
 }  // namespace clv
