@@ -964,11 +964,9 @@ int pick_rows(int X, int R, int max_sites) {
   return best;
 }
 
-// Sites per workgroup tile: 256, or CLV_TILE_SITES in [64, 256].
-int tile_sites() {
-  const char *e = std::getenv("CLV_TILE_SITES");
-  return e ? std::max(64, std::min(256, std::atoi(e))) : 256;
-}
+// Sites per workgroup tile.  Fixed: the tile sets the partial-sum
+// granularity, so results must not depend on anything but the shape.
+constexpr int kTileSites = 256;
 
 int dalloc(clv_ctx *ctx, Bucket &b, void **p, size_t bytes) {
   if (bytes == 0) bytes = 16;
@@ -1111,7 +1109,7 @@ int setup_bucket(clv_ctx *ctx, Bucket &b) {
   b.Vs = int64_t(sh.nx) * sh.ny * sh.nz;
   b.S = b.Vs * sh.nt;
   b.esz = (ctx->precision == CLV_FP32) ? 8 : 16;
-  b.rows = pick_rows(b.X, b.R, std::max(tile_sites(), b.X));
+  b.rows = pick_rows(b.X, b.R, std::max(kTileSites, b.X));
   // the update kernel stages 8 lines per row: keep its LDS within 64 KB
   const auto update_lds = [&](int rows) {
     return size_t(b.esz) * 2 * b.X + size_t(b.esz / 2) * (b.X + (b.X & 1)) +
