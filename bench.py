@@ -137,7 +137,7 @@ def run_uneven(args, log):
         p.eps = beta / s.nt
         eng.set_params(i, p)
         eng.init_fields(i, 1.0, 0.1)
-    eng.autotune(os.path.join(ROOT, "gpurun_out", "autotune_cache.jsonl") if args.tune_cache else None)
+    eng.autotune(args.tune_cache)
     eng.propagate(args.warmup, 10)
     eng.synchronize()
     torch.cuda.synchronize()
@@ -294,7 +294,8 @@ def main() -> int:
     ap.add_argument("--secant-steps", type=int, default=3000)
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-uneven", action="store_true")
-    ap.add_argument("--tune-cache", action="store_true")
+    ap.add_argument("--tune-cache", default=None, metavar="PATH",
+                    help="JSON-lines autotuner cache for the uneven batch (default: tune without a cache)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:

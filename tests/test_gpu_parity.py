@@ -2,7 +2,7 @@
 
 Tolerances (the bar the engine is held to):
   FP64: per-site |engine - oracle| <= 1e-10 after 1000 steps on the same Philox
-        stream (configs[0] shape), <= 1e-12 for single-step quantities.
+        stream (1-D 64 x 32), <= 1e-12 for single-step quantities.
   FP32: per-site |engine - oracle| <= 1e-4 * rms|oracle field| after 1000 steps.
 Quantities compared: drift statistics (max, mean of |D_0|), dt, Langevin time,
 fields after 1 and N steps (device controller and host-driven plain method),
@@ -126,7 +126,7 @@ def test_noise_free_update_matches(oracle_lib):
 
 @pytest.mark.parametrize("plain", [False, True])
 def test_trajectory_fp64_1000_steps(oracle_lib, plain):
-    """configs[0] shape (1-D 64 x 32), 1000 adaptive steps, per-site 1e-10."""
+    """1-D 64 x 32, 1000 adaptive steps, per-site 1e-10."""
     print("case: fp64 trajectory plain=%s" % plain, flush=True)
     shape = (64, 1, 1, 32)
     eng, orc, p = _pair(oracle_lib, shape, seed=1)

@@ -83,8 +83,9 @@ def test_fp32_observables_match_fp64_oracle(oracle_lib):
 
 
 def test_fp32_and_fp64_engine_chains_agree():
-    """configs[1]-like physics on a reduced lattice: FP32 and FP64 chains,
-    density / Sz / energy within two standard errors."""
+    """The 2-D configuration's couplings (mu = 1, c0 = 0.1, c1 = -0.05, q = 0.1)
+    on a reduced 16^2 x 16 lattice: FP32 and FP64 chains, density / Sz /
+    energy within two standard errors."""
     print("case: fp32 vs fp64 engine", flush=True)
     shape = Shape(16, 16, 1, 16)
     nch = 24
