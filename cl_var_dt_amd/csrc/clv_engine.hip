@@ -973,6 +973,7 @@ int dalloc(clv_ctx *ctx, Bucket &b, void **p, size_t bytes) {
   hipError_t e = hipMalloc(p, bytes);
   if (e != hipSuccess) {
     *p = nullptr;
+    (void)hipGetLastError();  // not left for a later launch check to report
     return fail(ctx, CLV_ERR_NOMEM, std::string("hipMalloc(") + std::to_string(bytes) +
                                         "): " + hipGetErrorString(e));
   }
@@ -1896,7 +1897,13 @@ int take_snapshot(clv_ctx *ctx, Bucket &b, Snapshot &sn) {
       {b.d_count, sizeof(int64_t) * b.nch}};
   for (auto &pr : parts) {
     void *copy = nullptr;
-    if (hipMalloc(&copy, pr.second) != hipSuccess) return fail(ctx, CLV_ERR_NOMEM, "autotune: no room for a state snapshot");
+    if (hipMalloc(&copy, pr.second) != hipSuccess) {
+      (void)hipGetLastError();
+      for (auto &sv : sn.saved) (void)hipFree(sv.second);  // the copies made so far
+      sn.saved.clear();
+      sn.sizes.clear();
+      return fail(ctx, CLV_ERR_NOMEM, "autotune: no room for a state snapshot");
+    }
     HIPCHK(ctx, hipMemcpyAsync(copy, pr.first, pr.second, hipMemcpyDeviceToDevice, ctx->stream));
     sn.saved.push_back({pr.first, copy});
     sn.sizes.push_back(pr.second);

@@ -110,3 +110,19 @@ def test_checkpoint_resume_is_exact(prec):
     assert s1.steps == s2.steps == 30 and s1.t_langevin == s2.t_langevin
     straight.close()
     resumed.close()
+
+
+def test_out_of_device_memory_is_an_error():
+    """A context larger than the device's HBM fails with an error code and
+    message (the first field buffer, 5 x 68.7 GB in FP64, cannot be had), frees
+    what it took, and leaves nothing behind for later launches to report."""
+    print("case: out of device memory", flush=True)
+    from cl_var_dt_amd.engine import Engine
+    with pytest.raises(ClvError, match="hipMalloc"):
+        Engine([Shape(128, 128, 128, 512)] * 5, precision=CLV_FP64)
+    e = _engine([(16, 2, 1, 4)])
+    e.set_params(0, _p(5))
+    e.init_fields(0, 1.0, 0.1)
+    e.propagate(6, 2)
+    assert e.stats(0).steps == 6 and e.stats(0).nonfinite == 0
+    e.close()
