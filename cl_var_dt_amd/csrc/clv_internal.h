@@ -99,11 +99,15 @@ struct clv_ctx {
   std::vector<GraphEntry> graphs;
 };
 
+// Reported once: the runtime's pending error is cleared, so a later launch
+// check does not report it again.
 #define HIPCHK(ctx, expr)                                                                  \
   do {                                                                                     \
     hipError_t e_ = (expr);                                                                \
-    if (e_ != hipSuccess)                                                                  \
+    if (e_ != hipSuccess) {                                                                \
+      (void)hipGetLastError();                                                             \
       return clvi::fail(ctx, CLV_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    }                                                                                      \
   } while (0)
 
 namespace clvi {
