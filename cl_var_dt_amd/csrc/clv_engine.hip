@@ -483,8 +483,13 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? CLV_FUSED_WAVES_PER_EU : 2) v
 // equal to the k-space sums of evaluate_n_* by Parseval), F_a(x)
 // (evaluate_spin_density_realspace, Observables_without_cutoff.cu:456-488)
 // and the energy density h on the time-split pair (psibar(t+1), psi(t)).
+// S = obs_slices(sizeof(C2<T>), X) slices per transform batch (two fill the
+// transform levels that one slice of a long row leaves mostly idle, with half
+// the barriers per slice).  Each thread still adds its slices in tau order,
+// so S does not change the result.
 template <typename T, int X>
 __global__ __launch_bounds__(256) void k_obs(BucketArgs<T> A) {
+  constexpr int S = obs_slices(int(sizeof(C2<T>)), X);
   extern __shared__ __align__(16) char smem_raw[];
   const int chain = A.chain0 + blockIdx.z;
   const ChainParams &P = A.prm[chain];
@@ -495,12 +500,13 @@ __global__ __launch_bounds__(256) void k_obs(BucketArgs<T> A) {
   const int tid = threadIdx.x, nthr = blockDim.x;
   const int j = tid / X, x = tid - (tid / X) * X;
   const bool has_site = j < A.rows;
-  Smem<T, X> sm(smem_raw, 4 * A.rows);
+  const int rows = A.rows, Nt = A.Nt, tc = A.tc;
+  Smem<T, X> sm(smem_raw, 4 * S * rows);
   load_tables<T, X>(sm, A.tw, A.p2, 1.0 / P.a_x, tid, nthr);
   const int par = st.parity;
   const C2<T> *psi = (par ? A.buf1_psi : A.buf0_psi) + chain * A.chain_stride;
   const C2<T> *psib = (par ? A.buf1_psib : A.buf0_psib) + chain * A.chain_stride;
-  const int r = tile * A.rows + j;
+  const int r = tile * rows + j;
   const double ax = P.a_x, G = P.G, c1 = P.c1;
   const double sz[4] = {P.P + P.Q, 0.0, 0.0, -P.P + P.Q};
   // interaction part of h through the matrix invariants of the pair
@@ -508,69 +514,82 @@ __global__ __launch_bounds__(256) void k_obs(BucketArgs<T> A) {
   //   = (G + c1/2) N^2 - (2G + 2c1) det Psibar det Psi   (N = sum_c psib_c psi_c)
   const double gN = G + 0.5 * c1, gD = 2.0 * G + 2.0 * c1;
   double n[4] = {0, 0, 0, 0}, e = 0.0, F[3] = {0, 0, 0};
-  // the slice's fields are loaded one slice ahead, overlapping the transform
   auto site_at = [&](int t) { return (int64_t(t) * A.R + r) * X + x; };
-  C2<T> q[4], b[4];
-  if (has_site) {
-    const int t = chunk * A.tc;
-    load4(psi, site_at(t), q);
-    load4(psib, site_at(t == A.Nt - 1 ? 0 : t + 1), b);
-  }
-  for (int tt = 0; tt < A.tc; ++tt) {
-    const int t = chunk * A.tc + tt;
-    C2<T> qn[4], bn[4];
+  // psi(t) and psibar(t+1) of the slices t .. t+cnt-1
+  auto load_pair = [&](int t, int cnt, C2<T> qd[S][4], C2<T> bd[S][4]) {
+#pragma unroll
+    for (int u = 0; u < S; ++u) {
+      if (u >= cnt) break;
+      const int tu = t + u;
+      load4(psi, site_at(tu), qd[u]);
+      load4(psib, site_at(tu + 1 == Nt ? 0 : tu + 1), bd[u]);
+    }
+  };
+  // a batch's fields are loaded one batch ahead, overlapping the transform
+  C2<T> q[S][4], b[S][4];
+  if (has_site) load_pair(chunk * tc, tc < S ? tc : S, q, b);
+  for (int tt = 0; tt < tc; tt += S) {
+    const int t = chunk * tc + tt, cnt = tc - tt < S ? tc - tt : S;
+    C2<T> qn[S][4], bn[S][4];
     if (has_site) {
 #pragma unroll
-      for (int c = 0; c < 4; ++c) sm.at(c * A.rows + j, x) = q[c];
-      if (tt + 1 < A.tc) {
-        load4(psi, site_at(t + 1), qn);
-        load4(psib, site_at(t + 2 >= A.Nt ? t + 2 - A.Nt : t + 2), bn);
+      for (int u = 0; u < S; ++u) {
+        if (u >= cnt) break;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) sm.at((4 * u + c) * rows + j, x) = q[u][c];
       }
+      if (tt + S < tc) load_pair(t + S, tc - tt - S < S ? tc - tt - S : S, qn, bn);
     }
     __syncthreads();
-    RowFFT<T, X>::kinetic(sm.buf, sm.tw, sm.mult, 4 * A.rows, tid, nthr);
+    RowFFT<T, X>::kinetic(sm.buf, sm.tw, sm.mult, 4 * cnt * rows, tid, nthr);
     if (has_site) {
-      C2<double> bb[4], qq[4], kk[4];
 #pragma unroll
-      for (int c = 0; c < 4; ++c) {
-        const C2<T> kc = sm.at(c * A.rows + j, x);
-        bb[c] = {double(b[c].x), double(b[c].y)};
-        qq[c] = {double(q[c].x), double(q[c].y)};
-        kk[c] = {double(kc.x), double(kc.y)};
-        if (has_kt(A, P)) {
-          const C2<T> kt = load1(A.kt_psi + chain * A.chain_stride, site_at(t), c);
-          kk[c].x += double(kt.x);
-          kk[c].y += double(kt.y);
+      for (int u = 0; u < S; ++u) {
+        if (u >= cnt) break;
+        C2<double> bb[4], qq[4], kk[4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          const C2<T> kc = sm.at((4 * u + c) * rows + j, x);
+          bb[c] = {double(b[u][c].x), double(b[u][c].y)};
+          qq[c] = {double(q[u][c].x), double(q[u][c].y)};
+          kk[c] = {double(kc.x), double(kc.y)};
+          if (has_kt(A, P)) {
+            const C2<T> kt = load1(A.kt_psi + chain * A.chain_stride, site_at(t + u), c);
+            kk[c].x += double(kt.x);
+            kk[c].y += double(kt.y);
+          }
         }
-      }
-      // only Re h is accumulated
-      C2<double> nn = {0.0, 0.0};
-      double hr = 0.0, bq_re[4];
+        // only Re h is accumulated
+        C2<double> nn = {0.0, 0.0};
+        double hr = 0.0, bq_re[4];
 #pragma unroll
-      for (int c = 0; c < 4; ++c) {
-        const C2<double> bq = cmul(bb[c], qq[c]);
-        bq_re[c] = bq.x;
-        n[c] += bq.x * ax;
-        nn = cadd(nn, bq);
-        hr -= (bb[c].x * kk[c].x - bb[c].y * kk[c].y) + sz[c] * bq.x;
+        for (int c = 0; c < 4; ++c) {
+          const C2<double> bq = cmul(bb[c], qq[c]);
+          bq_re[c] = bq.x;
+          n[c] += bq.x * ax;
+          nn = cadd(nn, bq);
+          hr -= (bb[c].x * kk[c].x - bb[c].y * kk[c].y) + sz[c] * bq.x;
+        }
+        const C2<double> detb = csub(cmul(bb[0], bb[3]), cmul(bb[1], bb[2]));
+        const C2<double> detq = csub(cmul(qq[0], qq[3]), cmul(qq[1], qq[2]));
+        hr += gN * (nn.x * nn.x - nn.y * nn.y) - gD * (detb.x * detq.x - detb.y * detq.y);
+        e += hr * ax;
+        // Re F_x, Re F_y = -Im(psib_2 psi_0 + psib_3 psi_1 - psib_0 psi_2 - psib_1 psi_3), Re F_z
+        F[0] += (bb[2].x * qq[0].x - bb[2].y * qq[0].y) + (bb[0].x * qq[2].x - bb[0].y * qq[2].y) +
+                (bb[3].x * qq[1].x - bb[3].y * qq[1].y) + (bb[1].x * qq[3].x - bb[1].y * qq[3].y);
+        F[1] -= (bb[2].x * qq[0].y + bb[2].y * qq[0].x) + (bb[3].x * qq[1].y + bb[3].y * qq[1].x) -
+                (bb[0].x * qq[2].y + bb[0].y * qq[2].x) - (bb[1].x * qq[3].y + bb[1].y * qq[3].x);
+        F[2] += (bq_re[0] + bq_re[1]) - (bq_re[2] + bq_re[3]);
       }
-      const C2<double> detb = csub(cmul(bb[0], bb[3]), cmul(bb[1], bb[2]));
-      const C2<double> detq = csub(cmul(qq[0], qq[3]), cmul(qq[1], qq[2]));
-      hr += gN * (nn.x * nn.x - nn.y * nn.y) - gD * (detb.x * detq.x - detb.y * detq.y);
-      e += hr * ax;
-      // Re F_x, Re F_y = -Im(psib_2 psi_0 + psib_3 psi_1 - psib_0 psi_2 - psib_1 psi_3), Re F_z
-      F[0] += (bb[2].x * qq[0].x - bb[2].y * qq[0].y) + (bb[0].x * qq[2].x - bb[0].y * qq[2].y) +
-              (bb[3].x * qq[1].x - bb[3].y * qq[1].y) + (bb[1].x * qq[3].x - bb[1].y * qq[3].y);
-      F[1] -= (bb[2].x * qq[0].y + bb[2].y * qq[0].x) + (bb[3].x * qq[1].y + bb[3].y * qq[1].x) -
-              (bb[0].x * qq[2].y + bb[0].y * qq[2].x) - (bb[1].x * qq[3].y + bb[1].y * qq[3].x);
-      F[2] += (bq_re[0] + bq_re[1]) - (bq_re[2] + bq_re[3]);
     }
     __syncthreads();
 #pragma unroll
-    for (int c = 0; c < 4; ++c) {
-      q[c] = qn[c];
-      b[c] = bn[c];
-    }
+    for (int u = 0; u < S; ++u)
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        q[u][c] = qn[u][c];
+        b[u][c] = bn[u][c];
+      }
   }
   if (has_site) {
     const int64_t vs = int64_t(x) * A.R + r;  // reference spatial order (x, y, z)
@@ -783,7 +802,7 @@ void launch_row_kernels(const Bucket &b, hipStream_t s, int which, int chain0, i
     hipLaunchKernelGGL((k_update<T, X>), dim3(b.ntiles, b.Nt, nch), block, sh, s, A);
   } else if (which == 2) {
     BucketArgs<T> A = make_args<T>(b, chain0, eval_every);
-    const size_t sh = Smem<T, X>::bytes(4 * b.rows);
+    const size_t sh = Smem<T, X>::bytes(4 * obs_slices(int(sizeof(C2<T>)), X) * b.rows);
     hipLaunchKernelGGL((k_obs<T, X>), dim3(b.ntiles, b.nchunks, nch), block, sh, s, A);
   } else {
     BucketArgs<T> A = make_args<T>(b, chain0, eval_every, true);

@@ -40,6 +40,15 @@ struct BucketArgs {
                                 // < 0: each chain's own ChainParams::eval_every; 0: always
 };
 
+// tau slices per transform batch of k_obs.  One slice of X >= 64 is at most
+// 16 lines and leaves transform levels idle; two fill them but hold twice the
+// fields in registers.  Measured on MI355X, two are faster for FP32 at
+// X >= 96 (6-15%) and FP64 at X = 64, 96, 256 (3-23%); FP32 at X = 64 is 3%
+// slower, and FP64 at X = 128, 192 drops to one wave per SIMD (33% slower).
+__host__ __device__ constexpr int obs_slices(int complex_bytes, int X) {
+  return complex_bytes == 8 ? (X >= 96 ? 2 : 1) : ((X == 64 || X == 96 || X == 256) ? 2 : 1);
+}
+
 // Does this chain evaluate its observables after the step just applied?
 template <typename T>
 __device__ __forceinline__ bool evaluates(const BucketArgs<T> &A, const ChainParams &P,
