@@ -226,12 +226,7 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? 5 : 2) void k_update(BucketAr
   }
   const T dtT = T(dt);
   C2<T> o[4], ob[4];
-#pragma unroll
-  for (int c = 0; c < 4; ++c) {
-    const C2<T> eta = {nz[2 * c], nz[2 * c + 1]};
-    o[c] = cadd(p0[c], cadd(cscale(dtT, D[c]), eta));
-    ob[c] = cadd(b0[c], cadd(cscale(dtT, Db[c]), cconj(eta)));
-  }
+  update_site(dtT, p0, b0, D, Db, nz, o, ob);
   store4(opsi, s0, o);
   store4(opsib, s0, ob);
 }
@@ -414,12 +409,7 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? CLV_FUSED_WAVES_PER_EU : 2) v
         normals4(nz + 4, P.seed, 0u, step, site, 1u, sq);
       }
       C2<T> o[4], ob[4];
-#pragma unroll
-      for (int c = 0; c < 4; ++c) {
-        const C2<T> eta = {nz[2 * c], nz[2 * c + 1]};
-        o[c] = cadd(p0[c], cadd(cscale(dtT, D[c]), eta));
-        ob[c] = cadd(b0[c], cadd(cscale(dtT, Db[c]), cconj(eta)));
-      }
+      update_site(dtT, p0, b0, D, Db, nz, o, ob);
       const int64_t s0 = site_at(t);
       store4(opsi, s0, o);
       store4(opsib, s0, ob);

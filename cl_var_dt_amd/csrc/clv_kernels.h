@@ -155,13 +155,45 @@ __device__ __forceinline__ C2<T> cdet(const C2<T> a[4]) {
           a[0].x * a[3].y + a[0].y * a[3].x - (a[1].x * a[2].y + a[1].y * a[2].x)};
 }
 
+// a * b + c with one rounding, in the field precision
+template <typename T>
+__device__ __forceinline__ T fmad(T a, T b, T c) {
+  if constexpr (sizeof(T) == 4) return __builtin_fmaf(a, b, c);
+  else return __builtin_fma(a, b, c);
+}
+
 // One component of a drift: af (f1 - f0) + kin + c f1' + B cof, with
-// f1' = f1 except in the psibar equation of the reference Q3 mode.
+// f1' = f1 except in the psibar equation of the reference Q3 mode.  Written
+// as one chain of FMAs (the complex products' negations are free operand
+// modifiers), six instructions per real part.
 template <typename T>
 __device__ __forceinline__ C2<T> drift_term(T af, C2<T> f1, C2<T> f0, C2<T> kin, C2<T> c, C2<T> f1c,
                                             C2<T> B, C2<T> cof) {
-  return {af * (f1.x - f0.x) + kin.x + (c.x * f1c.x - c.y * f1c.y) + (B.x * cof.x - B.y * cof.y),
-          af * (f1.y - f0.y) + kin.y + (c.x * f1c.y + c.y * f1c.x) + (B.x * cof.y + B.y * cof.x)};
+  T x = fmad(af, f1.x - f0.x, kin.x);
+  x = fmad(c.x, f1c.x, x);
+  x = fmad(-c.y, f1c.y, x);
+  x = fmad(B.x, cof.x, x);
+  x = fmad(-B.y, cof.y, x);
+  T y = fmad(af, f1.y - f0.y, kin.y);
+  y = fmad(c.x, f1c.y, y);
+  y = fmad(c.y, f1c.x, y);
+  y = fmad(B.x, cof.y, y);
+  y = fmad(B.y, cof.x, y);
+  return {x, y};
+}
+
+// psi + (dt D + eta) and psibar + (dt Db + conj eta) at one site (add_noise,
+// multiply_by_timestep and operator+=, Propagator_without_cutoff.cu:131-157,
+// 434-441); noise nz = (re, im) of eta per component.
+template <typename T>
+__device__ __forceinline__ void update_site(T dt, const C2<T> p0[4], const C2<T> b0[4], const C2<T> D[4],
+                                            const C2<T> Db[4], const T nz[8], C2<T> o[4], C2<T> ob[4]) {
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    const T ex = nz[2 * c], ey = nz[2 * c + 1];
+    o[c] = {p0[c].x + fmad(dt, D[c].x, ex), p0[c].y + fmad(dt, D[c].y, ey)};
+    ob[c] = {b0[c].x + fmad(dt, Db[c].x, ex), b0[c].y + fmad(dt, Db[c].y, -ey)};
+  }
 }
 
 // Drift of psi and psibar at one site, all four components
@@ -226,8 +258,17 @@ __device__ __forceinline__ C2<T> drift0_local(const Coef<T> &k, const C2<T> pm[4
   const C2<T> B = {k.kB * det.x, k.kB * det.y};
   const C2<T> c = {k.cs[0] - k.kA * N.x, -(k.kA * N.y)};
   const C2<T> f1 = pm[0], cof = b0[3];
-  return {k.af * (f1.x - p00.x) + (c.x * f1.x - c.y * f1.y) + (B.x * cof.x - B.y * cof.y),
-          k.af * (f1.y - p00.y) + (c.x * f1.y + c.y * f1.x) + (B.x * cof.y + B.y * cof.x)};
+  T x = k.af * (f1.x - p00.x);
+  x = fmad(c.x, f1.x, x);
+  x = fmad(-c.y, f1.y, x);
+  x = fmad(B.x, cof.x, x);
+  x = fmad(-B.y, cof.y, x);
+  T y = k.af * (f1.y - p00.y);
+  y = fmad(c.x, f1.y, y);
+  y = fmad(c.y, f1.x, y);
+  y = fmad(B.x, cof.y, y);
+  y = fmad(B.y, cof.x, y);
+  return {x, y};
 }
 
 template <typename T>
