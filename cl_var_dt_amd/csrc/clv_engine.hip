@@ -1225,7 +1225,7 @@ void step_bucket(clv_ctx *ctx, Bucket &b, int eval_every) {
   }
 }
 
-void enqueue_step_gated(clv_ctx *ctx, Bucket &b, int eval_every) {
+void enqueue_step_gated(clv_ctx *ctx, Bucket &b, int eval_every, bool obs) {
   if (uses_fused(b)) {
     row_kernel(ctx, b, 3, 0, b.nch, 0);
     b.epoch ^= 1;
@@ -1235,7 +1235,7 @@ void enqueue_step_gated(clv_ctx *ctx, Bucket &b, int eval_every) {
     controller(ctx, b, 0, b.nch, 0, 0.0, 0);
     row_kernel(ctx, b, 1, 0, b.nch, 0);
   }
-  if (eval_every != 0) {
+  if (eval_every != 0 && obs) {
     transverse(ctx, b, 0, b.nch, 1, eval_every);
     row_kernel(ctx, b, 2, 0, b.nch, eval_every);
     obs_finish(ctx, b, 0, b.nch, eval_every);
@@ -1598,12 +1598,40 @@ int clv_init_fields(clv_ctx *ctx, int32_t chain, double n0, double var) {
   return CLV_OK;
 }
 
-// Captured graph of two steps of every bucket (two steps so that the fused
-// schedule's alternating state slots return to the captured parity).
-static int graph_for(clv_ctx *ctx, int eval_every, hipGraphExec_t *out) {
-  std::string key = std::to_string(eval_every);
-  for (const Bucket &b : ctx->buckets)
+// The step count of every active chain of b when they are all equal, else -1.
+static int64_t common_steps(const Bucket &b) {
+  int64_t s = -1;
+  for (int i = 0; i < b.nch; ++i) {
+    if (!b.host_active[i]) continue;
+    if (s >= 0 && b.host_steps[i] != s) return -1;
+    s = b.host_steps[i];
+  }
+  return s < 0 ? 0 : s;
+}
+
+// Captured graph of an even number of steps of every bucket (even so that the
+// fused schedule's alternating state slots return to the captured parity).
+// With a uniform cadence (eval_every > 0, at most 32) and every bucket's
+// chains at one step count, the graph spans a cadence period and launches
+// the observable kernels only after the steps that evaluate (the phase is
+// part of the key); otherwise two steps with the observable kernels after
+// each, gated per chain on the device.  *steps: steps per replay.
+static int graph_for(clv_ctx *ctx, int eval_every, hipGraphExec_t *out, int *steps) {
+  bool phased = eval_every > 0 && eval_every <= 32;
+  std::vector<int64_t> phase;
+  for (const Bucket &b : ctx->buckets) {
+    const int64_t s = common_steps(b);
+    if (s < 0) phased = false;
+    phase.push_back(s < 0 ? 0 : s % (eval_every > 0 ? eval_every : 1));
+  }
+  const int G = phased ? (eval_every % 2 == 0 ? eval_every : 2 * eval_every) : 2;
+  std::string key = std::to_string(eval_every) + (phased ? "p" : "g");
+  for (size_t i = 0; i < ctx->buckets.size(); ++i) {
+    const Bucket &b = ctx->buckets[i];
     key += "/" + std::to_string(b.schedule) + ":" + std::to_string(b.epoch & 1);
+    if (phased) key += ":" + std::to_string(phase[i]);
+  }
+  *steps = G;
   for (auto &g : ctx->graphs)
     if (g.key == key) {
       *out = g.exec;
@@ -1612,8 +1640,10 @@ static int graph_for(clv_ctx *ctx, int eval_every, hipGraphExec_t *out) {
   std::vector<int> epochs;
   for (const Bucket &b : ctx->buckets) epochs.push_back(b.epoch);
   HIPCHK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-  for (int n = 0; n < 2; ++n)
-    for (Bucket &b : ctx->buckets) enqueue_step_gated(ctx, b, eval_every);
+  for (int n = 0; n < G; ++n)
+    for (size_t i = 0; i < ctx->buckets.size(); ++i)
+      enqueue_step_gated(ctx, ctx->buckets[i], eval_every,
+                         !phased || (phase[i] + n) % eval_every == 0);
   hipGraph_t graph = nullptr;
   const hipError_t ec = hipStreamEndCapture(ctx->stream, &graph);
   for (size_t i = 0; i < ctx->buckets.size(); ++i) ctx->buckets[i].epoch = epochs[i];
@@ -1644,15 +1674,16 @@ int clv_propagate(clv_ctx *ctx, int64_t nsteps, int32_t eval_every) {
       b.primed = true;
     }
   hipGraphExec_t exec = nullptr;
-  int rc = graph_for(ctx, eval_every, &exec);
+  int G = 2;
+  int rc = graph_for(ctx, eval_every, &exec, &G);
   if (rc) return rc;
-  const int64_t pairs = nsteps / 2;
-  for (int64_t k = 0; k < pairs; ++k) HIPCHK(ctx, hipGraphLaunch(exec, ctx->stream));
+  const int64_t reps = nsteps / G;
+  for (int64_t k = 0; k < reps; ++k) HIPCHK(ctx, hipGraphLaunch(exec, ctx->stream));
   for (Bucket &b : ctx->buckets) {
-    count_step(b, 2 * pairs);
+    count_step(b, G * reps);
     if (!(uses_fused(b))) b.primed = false;
   }
-  for (int64_t n = 2 * pairs; n < nsteps; ++n)
+  for (int64_t n = G * reps; n < nsteps; ++n)
     for (Bucket &b : ctx->buckets) step_bucket(ctx, b, eval_every);
   HIPCHK(ctx, hipGetLastError());
   return CLV_OK;

@@ -92,8 +92,8 @@ struct clv_ctx {
   std::vector<hipEvent_t> event_pool;
   double kt_ms[7] = {0, 0, 0, 0, 0, 0, 0};
   int64_t kt_n[7] = {0, 0, 0, 0, 0, 0, 0};
-  // captured two-step graphs of clv_propagate, keyed by the eval cadence and
-  // the buckets' schedules and state-slot parities at capture time
+  // captured step graphs of clv_propagate, keyed by the eval cadence and the
+  // buckets' schedules, state-slot parities and cadence phases at capture time
   bool use_graphs = true;
   struct GraphEntry { std::string key; hipGraphExec_t exec; };
   std::vector<GraphEntry> graphs;
@@ -123,9 +123,9 @@ ChainParams to_device_params(const clv_params &p, int eval_every, int active,
 // One Langevin step of every active chain of the bucket; eval_every < 0
 // evaluates each chain at its own cadence (ChainParams::eval_every).
 void step_bucket(clv_ctx *ctx, Bucket &b, int eval_every);
-// Launches of one step of bucket b with the observable kernels always
-// enqueued and gated on the device (graph capture); no host bookkeeping.
-void enqueue_step_gated(clv_ctx *ctx, Bucket &b, int eval_every);
+// Launches of one step of bucket b for graph capture; the observable kernels
+// (gated per chain on the device) follow when `obs`; no host bookkeeping.
+void enqueue_step_gated(clv_ctx *ctx, Bucket &b, int eval_every, bool obs = true);
 void drop_graphs(clv_ctx *ctx);
 int read_state(clv_ctx *ctx, const Bucket &b, int local, ChainState *out);
 // Serving helpers, all ordered on the context's stream.
