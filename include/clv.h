@@ -195,8 +195,11 @@ int clv_get_schedule(const clv_ctx *ctx, int32_t chain);
  * Results do not depend on either; a combination that does not fit 64 KB of
  * LDS for some shape is rejected.  clv_autotune times them. */
 int clv_set_fused_options(clv_ctx *ctx, int32_t line_sets, int32_t batch);
-/* clv_propagate replays a captured HIP graph of two steps of every bucket
- * (on by default); off runs every step eagerly.  Results are identical. */
+/* clv_propagate replays captured HIP graphs of its steps (on by default): a
+ * cadence period with the observable kernels only where they evaluate when
+ * eval_every > 0 and each bucket's chains share a step count, else two steps
+ * with the observable kernels gated per chain; off runs every step eagerly.
+ * Results are identical. */
 int clv_set_graphs(clv_ctx *ctx, int32_t on);
 
 /* ---- kernel timing -----------------------------------------------------
