@@ -459,8 +459,10 @@ def test_fused_fp64_short_tau_matches_oracle(oracle_lib, shape):
 
 @pytest.mark.parametrize("sched", [1, 2])
 def test_graph_replay_equals_eager(sched):
-    """clv_propagate's captured two-step graph gives the eager result bit for bit,
-    replay after replay, with eager work and synchronisations in between."""
+    """clv_propagate's captured graphs give the eager result bit for bit,
+    replay after replay, with eager work and synchronisations in between
+    (cadence 3: graphs of a six-step period with the observables after the
+    steps that evaluate)."""
     print("case: graph replay", sched, flush=True)
     shapes = [(32, 8, 1, 8), (16, 4, 2, 4), (64, 2, 1, 16)]
     engines = []
@@ -488,6 +490,45 @@ def test_graph_replay_equals_eager(sched):
             assert g.stats(i).t_langevin == eager.stats(i).t_langevin
         g.evaluate(0)
         eager.evaluate(0)
+    for e in engines:
+        e.close()
+
+
+@pytest.mark.parametrize("sched", [1, 2])
+def test_graphs_with_chains_at_different_steps(sched):
+    """Chains of one bucket at different step counts (one restored at step 5):
+    the graphs fall back to two steps with the observables gated per chain, and
+    fields, step counts and every chain's observables equal eager steps bit for
+    bit, with cadences 4 and per-chain phases that differ."""
+    print("case: graphs, chains at different steps", sched, flush=True)
+    shapes = [(32, 4, 1, 8)] * 3
+    engines = []
+    for graphs in (True, False):
+        e = _engine(shapes, CLV_FP32)
+        for i in range(3):
+            e.set_params(i, _params(seed=70 + i, mu=0.8))
+            e.init_fields(i, 1.0, 0.1)
+        st = e.stats(1)
+        st.steps = 5
+        e.set_stats(1, st)
+        e.set_schedule(sched)
+        e.set_graphs(graphs)
+        engines.append(e)
+    g, eager = engines
+    g.propagate(14, 4)
+    for _ in range(14):
+        eager.propagate(1, 4)
+    for i in range(3):
+        a, _ = g.get_fields(i)
+        b, _ = eager.get_fields(i)
+        assert np.array_equal(a, b), i
+        oa, Fa = g.observables(i)
+        ob, Fb = eager.observables(i)
+        assert oa.count == ob.count and oa.n_tot == ob.n_tot and oa.energy == ob.energy, i
+        assert all(np.array_equal(x, y) for x, y in zip(Fa, Fb)), i
+        assert g.stats(i).steps == eager.stats(i).steps == (19 if i == 1 else 14)
+    # 14 steps at cadence 4 from step 0: steps 0, 4, 8, 12; chain 1 from 5: 8, 12, 16
+    assert g.observables(0)[0].count == 4 and g.observables(1)[0].count == 3
     for e in engines:
         e.close()
 
