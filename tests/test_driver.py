@@ -104,3 +104,26 @@ def test_driver_run_matches_oracle(tmp_path, oracle_lib):
     rows.append([obs.n_tot, *obs.n_comp])
     got = np.array([[float(v) for v in row[:5]] for row in sk])
     assert np.allclose(got, np.array(rows), atol=2e-6)
+
+
+def test_integration_binding_compiles(tmp_path):
+    """The reference-side C++ binding shown in INTEGRATION.md (Option B: the
+    reference's Propagator members over clv.h) compiles and links against
+    libclv.so as written."""
+    lib = os.path.join(ROOT, "cl_var_dt_amd", "libclv.so")
+    if not os.path.exists(lib):
+        pytest.skip("libclv.so not built")
+    doc = open(os.path.join(ROOT, "INTEGRATION.md")).read()
+    start = doc.index("```cpp\n") + len("```cpp\n")
+    code = doc[start:doc.index("```", start)]
+    src = tmp_path / "binding.cpp"
+    src.write_text("#include <iostream>\n"
+                   "#define XSIZE 64\n#define YSIZE 1\n#define ZSIZE 1\n#define TAUSIZE 32\n"
+                   "static const double G = 0.1, Q = 0.1, EPS = 1.0 / 32, a_x = 1.0, DT = 1e-3;\n"
+                   + code +
+                   "\nint main() {\n  Propagator p(1234ULL, 0.0, 1.0);\n  p.propagate(0.0, 1.0);\n"
+                   "  p.propagate_without_noise(0.5, 0.0, 1.0);\n  return 0;\n}\n")
+    r = subprocess.run(["g++", "-std=c++17", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"),
+                        "-o", str(tmp_path / "binding"), str(src), "-L", os.path.dirname(lib), "-lclv"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
