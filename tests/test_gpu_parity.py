@@ -124,6 +124,30 @@ def test_noise_free_update_matches(oracle_lib):
     eng.close()
 
 
+@pytest.mark.parametrize("sched", [1, 2])
+def test_propagate_without_noise_matches(oracle_lib, sched):
+    """propagate_without_noise (Propagator_without_cutoff.cu:444-477): the
+    drift scaled by factor = 0.5, its max/mean and the adaptive dt taken from
+    that scaled drift, no noise; 30 steps of each schedule against the oracle
+    (per site 1e-11, dt and max |D_0| to 1e-11 relative)."""
+    print("case: propagate without noise", sched, flush=True)
+    shape = (32, 4, 1, 8)
+    eng, orc, p = _pair(oracle_lib, shape, noise_on=0, factor=0.5, seed=12)
+    eng.set_schedule(sched)
+    eng.init_fields(0, 1.0, 0.1)
+    orc.init_fields(1.0, 0.1)
+    eng.propagate(30, 0)
+    orc.propagate(30, 0)
+    a, b = eng.get_fields(0)
+    c, d = orc.get_fields()
+    assert max(np.max(abs(a - c)), np.max(abs(b - d))) < 1e-11
+    sg, so = eng.stats(0), orc.stats()
+    assert sg.steps == so.steps == 30
+    assert sg.dt == pytest.approx(so.dt, rel=1e-11)
+    assert sg.max_drift == pytest.approx(so.max_drift, rel=1e-11)
+    eng.close()
+
+
 @pytest.mark.parametrize("plain", [False, True])
 def test_trajectory_fp64_1000_steps(oracle_lib, plain):
     """1-D 64 x 32, 1000 adaptive steps, per-site 1e-10."""
