@@ -67,9 +67,11 @@ inline bool uses_fused(const Bucket &b) { return b.schedule == 2 && b.fused_lds 
 // Schedule without timing: the fused march has one workgroup per (tile, chain)
 // and walks tau serially, so it only pays off with many of them; below that the
 // two-pass kernels (one workgroup per tile and tau) fill the GPU better.  The
-// threshold is measured on MI355X (level at ~1k workgroups, fused ahead above).
-// Both schedules give identical results, so the choice may change at any step.
-constexpr int kFusedMinWorkgroups = 1024;
+// threshold is measured on MI355X: at 512 workgroups fused is 0.80-0.99 of the
+// two-pass time across 2-D and 3-D shapes and Nt = 16..128, at 320-384 it is
+// 1.05-1.33.  Both schedules give identical results, so the choice may change
+// at any step.
+constexpr int kFusedMinWorkgroups = 512;
 inline int auto_schedule(const Bucket &b, int nactive) {
   return (b.fused_lds > 0 && !b.kt_on && int64_t(b.ntiles) * nactive >= kFusedMinWorkgroups) ? 2 : 1;
 }
