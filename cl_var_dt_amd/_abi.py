@@ -19,8 +19,8 @@ CLV_ERR_NOTFOUND = -6
 
 CLV_FP64 = 0
 CLV_FP32 = 1
-CLV_Q3_REFERENCE = 0
-CLV_Q3_PSIBAR = 1
+CLV_CONJ_DRIFT_REFERENCE = 0
+CLV_CONJ_DRIFT_ACTION = 1
 
 SCHED_AUTO = 0
 SCHED_TWO_PASS = 1
@@ -55,7 +55,7 @@ class Params(C.Structure):
         ("Q", C.c_double), ("eps", C.c_double), ("a_x", C.c_double), ("DT", C.c_double),
         ("dt_min", C.c_double), ("dt_max", C.c_double), ("alpha", C.c_double),
         ("factor", C.c_double), ("seed", C.c_uint64), ("kin_dims", C.c_int32),
-        ("q3_mode", C.c_int32), ("noise_on", C.c_int32), ("reserved", C.c_int32),
+        ("conj_drift_mode", C.c_int32), ("noise_on", C.c_int32), ("reserved", C.c_int32),
     ]
 
 
@@ -105,7 +105,7 @@ def default_params(**kw) -> Params:
     """Reference defaults (Parameters.h:13-31, Propagator_without_cutoff.cu:391-402)."""
     p = Params(mu=0.0, P=0.0, G=0.1, c1=0.0, Q=0.0, eps=0.01, a_x=1.0, DT=0.002,
                dt_min=1e-8, dt_max=1e-2, alpha=0.9, factor=1.0, seed=0,
-               kin_dims=1, q3_mode=CLV_Q3_REFERENCE, noise_on=1, reserved=0)
+               kin_dims=1, conj_drift_mode=CLV_CONJ_DRIFT_REFERENCE, noise_on=1, reserved=0)
     for k, v in kw.items():
         if not hasattr(p, k):
             raise AttributeError(f"clv_params has no field {k!r}")

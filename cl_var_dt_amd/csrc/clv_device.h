@@ -36,7 +36,7 @@ __device__ __forceinline__ C2<T> cmul_si(C2<T> a) { return {T(-S) * a.y, T(S) * 
 struct ChainParams {
   double mu, P, G, c1, Q, eps, a_x, DT, dt_min, dt_max, alpha, factor;
   uint64_t seed;
-  int32_t q3_mode, noise_on, active;
+  int32_t conj_drift_mode, noise_on, active;
   int32_t kin_dims;    // 1: p^2 = p_x^2; 3: full Laplacian (transverse term in kt)
   int32_t eval_every;  // per-chain observable cadence (serving); 0 never
   int32_t pad;

@@ -1044,7 +1044,7 @@ ChainParams to_device_params(const clv_params &p, int eval_every, int active, in
   ChainParams d;
   d.mu = p.mu; d.P = p.P; d.G = p.G; d.c1 = p.c1; d.Q = p.Q; d.eps = p.eps; d.a_x = p.a_x;
   d.DT = p.DT; d.dt_min = p.dt_min; d.dt_max = p.dt_max; d.alpha = p.alpha; d.factor = p.factor;
-  d.seed = p.seed; d.q3_mode = p.q3_mode; d.noise_on = p.noise_on; d.active = active;
+  d.seed = p.seed; d.conj_drift_mode = p.conj_drift_mode; d.noise_on = p.noise_on; d.active = active;
   d.kin_dims = p.kin_dims;
   d.eval_every = eval_every;
   d.pad = 0;
@@ -1377,7 +1377,7 @@ void clv_default_params(clv_params *p) {
   p->factor = 1.0;
   p->seed = 0;
   p->kin_dims = 1;
-  p->q3_mode = CLV_Q3_REFERENCE;
+  p->conj_drift_mode = CLV_CONJ_DRIFT_REFERENCE;
   p->noise_on = 1;
 }
 
@@ -1486,8 +1486,8 @@ int clv_set_params(clv_ctx *ctx, int32_t chain, const clv_params *p) {
     return fail(ctx, CLV_ERR_ARG,
                 "clv_set_params: kin_dims must be 1 (p^2 = p_x^2, the reference's kinetic term) "
                 "or 3 (p_x^2 + p_y^2 + p_z^2)");
-  if (p->q3_mode != CLV_Q3_REFERENCE && p->q3_mode != CLV_Q3_PSIBAR)
-    return fail(ctx, CLV_ERR_ARG, "clv_set_params: bad q3_mode");
+  if (p->conj_drift_mode != CLV_CONJ_DRIFT_REFERENCE && p->conj_drift_mode != CLV_CONJ_DRIFT_ACTION)
+    return fail(ctx, CLV_ERR_ARG, "clv_set_params: bad conj_drift_mode");
   if (!(p->a_x > 0.0) || !(p->DT > 0.0) || !(p->eps > 0.0))
     return fail(ctx, CLV_ERR_ARG, "clv_set_params: a_x, DT and eps must be positive");
   (void)hipSetDevice(ctx->device);

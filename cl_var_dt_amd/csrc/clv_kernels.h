@@ -104,7 +104,7 @@ struct Coef {
   T cs[4];    // a_x factor eps s_i,  s = (mu+P+Q, mu, mu, mu-P+Q)
   T kA, kB;   // a_x factor eps (2G + c1),  a_x factor eps (2G + 2 c1)
   T kK;       // a_x factor eps (weight of the unscaled transverse kinetic term)
-  int q3;
+  int conj_drift;
 };
 
 template <typename T>
@@ -119,7 +119,7 @@ __device__ __forceinline__ Coef<T> coef(const ChainParams &p) {
   c.kA = T(e * (2.0 * p.G + p.c1));
   c.kB = T(e * (2.0 * p.G + 2.0 * p.c1));
   c.kK = T(e);
-  c.q3 = p.q3_mode;
+  c.conj_drift = p.conj_drift_mode;
   return c;
 }
 
@@ -163,7 +163,7 @@ __device__ __forceinline__ T fmad(T a, T b, T c) {
 }
 
 // One component of a drift: af (f1 - f0) + kin + c f1' + B cof, with
-// f1' = f1 except in the psibar equation of the reference Q3 mode.  Written
+// f1' = f1 except in the psibar equation of the reference conj-drift mode.  Written
 // as one chain of FMAs (the complex products' negations are free operand
 // modifiers), six instructions per real part.
 template <typename T>
@@ -198,10 +198,10 @@ __device__ __forceinline__ void update_site(T dt, const C2<T> p0[4], const C2<T>
 
 // Drift of psi and psibar at one site, all four components
 // (calculate_drift_without_noise, Propagator_without_cutoff.cu:1-129, with
-// factor = f) plus the spin term -c1 S_i and the Q3 switch:
+// factor = f) plus the spin term -c1 S_i and the conj-drift switch:
 //   D_i  = a_x f [psi_i(t-1) - psi_i(t) + eps (s_i psi_i(t-1) - 2G T-_i - c1 S-_i + K_i(t-1))]
 //   Db_i = a_x f [psib_i(t+1) - psib_i(t) + eps (s_i x_i - 2G T+_i - c1 S+_i + Kb_i(t+1))]
-// with x = psi(t+1) (reference, Q3) or psibar(t+1).  The interaction is
+// with x = psi(t+1) (the reference's :71,88,105,122) or psibar(t+1).  The interaction is
 // evaluated through the invariants of the 2x2 matrices Psi = [[psi_0, psi_1],
 // [psi_2, psi_3]] and Psibar on the time-split pair (Psi(t-1), Psibar(t)):
 // with n_j = psi_j psib_j, N = sum_j n_j, det = det Psi and the cofactors
@@ -232,7 +232,7 @@ __device__ __forceinline__ void drift_site(const Coef<T> &k, const C2<T> pm[4], 
     const C2<T> B = {k.kB * det.x, k.kB * det.y};
     const T ax = k.kA * N.x, ay = k.kA * N.y;
     const C2<T> cof[4] = {p0[3], {-p0[2].x, -p0[2].y}, {-p0[1].x, -p0[1].y}, p0[0]};
-    if (k.q3 == 1) {
+    if (k.conj_drift == 1) {
 #pragma unroll
       for (int i = 0; i < 4; ++i)
         Db[i] = drift_term(k.af, bp[i], b0[i], kp[i], C2<T>{k.cs[i] - ax, -ay}, bp[i], B, cof[i]);
@@ -247,7 +247,7 @@ __device__ __forceinline__ void drift_site(const Coef<T> &k, const C2<T> pm[4], 
   }
 }
 
-// Component 0 of the psi drift only (what the max/mean reduction reads, Q2),
+// Component 0 of the psi drift only (what the max/mean reduction reads),
 // in two parts: everything but the kinetic term, and then + K'_0(t-1) as the
 // last operation, so the fused march can form the first part at one tau and
 // add the kinetic term after a later, batched transform with the same bits.

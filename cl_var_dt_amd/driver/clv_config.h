@@ -50,7 +50,7 @@ inline std::map<std::string, std::string> read_defines(const std::string &filena
 
 // The compile-time configuration of the reference (Parameters.h:6-42), plus
 // the engine's extensions (C1, KIN_DIMS, PRECISION, INIT, INIT_N0, INIT_VAR,
-// Q3_MODE, DT_DEVICE), all with defaults that reproduce the reference.
+// CONJ_DRIFT_MODE, DT_DEVICE), all with defaults that reproduce the reference.
 struct RunConfig {
   int components = 4;
   int X = 256, Y = 1, Z = 1, Nt = 32;
@@ -63,7 +63,7 @@ struct RunConfig {
   double C1 = 0.0;
   int KIN_DIMS = 1;
   int PRECISION = 0;  // 0 FP64, 1 FP32
-  int Q3_MODE = 0;
+  int CONJ_DRIFT_MODE = 0;
   std::string INIT = "zero";  // zero | mean_field | gaussian
   double INIT_N0 = 1.0, INIT_VAR = 0.1;
   int DT_DEVICE = 0;  // 0: host-driven dt per step (reference); 1: device-resident dt
@@ -105,7 +105,7 @@ inline RunConfig config_from_defines(const std::map<std::string, std::string> &d
   num("C1", c.C1);
   inum("KIN_DIMS", c.KIN_DIMS);
   inum("PRECISION", c.PRECISION);
-  inum("Q3_MODE", c.Q3_MODE);
+  inum("CONJ_DRIFT_MODE", c.CONJ_DRIFT_MODE);
   auto it = d.find("INIT");
   if (it != d.end()) c.INIT = it->second;
   num("INIT_N0", c.INIT_N0);

@@ -290,7 +290,7 @@ int main(int argc, char **argv) {
   p.DT = cfg.DT;
   p.seed = seedvalue;
   p.kin_dims = cfg.KIN_DIMS;
-  p.q3_mode = cfg.Q3_MODE;
+  p.conj_drift_mode = cfg.CONJ_DRIFT_MODE;
 
   Lattice *lat = nullptr;
   Propagator *prop = nullptr;

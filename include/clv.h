@@ -39,11 +39,11 @@ extern "C" {
 #define CLV_FP64 0
 #define CLV_FP32 1
 
-/* Q3 of the reference: the psi-bar drift's potential term multiplies
- * psi(tau+1) (Propagator_without_cutoff.cu:71,88,105,122).  CLV_Q3_REFERENCE
- * keeps that; CLV_Q3_PSIBAR uses psibar(tau+1) (the action derivative). */
-#define CLV_Q3_REFERENCE 0
-#define CLV_Q3_PSIBAR 1
+/* Conj-drift mode.  In the reference the psi-bar drift's potential term multiplies
+ * psi(tau+1) (Propagator_without_cutoff.cu:71,88,105,122).  CLV_CONJ_DRIFT_REFERENCE
+ * keeps that; CLV_CONJ_DRIFT_ACTION uses psibar(tau+1) (the action derivative). */
+#define CLV_CONJ_DRIFT_REFERENCE 0
+#define CLV_CONJ_DRIFT_ACTION 1
 
 /* One chain's lattice: X x Y x Z space, Nt imaginary-time slices, 4 comps.
  * Stands in for the XSIZE/YSIZE/ZSIZE/TAUSIZE macros (Parameters.h:8-11). */
@@ -70,7 +70,7 @@ typedef struct clv_params {
   uint64_t seed;       /* Philox key of this chain's noise stream           */
   int32_t kin_dims;    /* 1: p^2 = p_x^2 (reference, :196); 3: px^2+py^2+pz^2 (two-pass
                           schedule; Y, Z in the supported lengths; not in the server) */
-  int32_t q3_mode;     /* CLV_Q3_REFERENCE or CLV_Q3_PSIBAR                 */
+  int32_t conj_drift_mode;     /* CLV_CONJ_DRIFT_REFERENCE or CLV_CONJ_DRIFT_ACTION                 */
   int32_t noise_on;    /* 1: propagate (:406); 0: propagate_without_noise (:444) */
   int32_t reserved;
 } clv_params;

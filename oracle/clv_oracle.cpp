@@ -207,7 +207,7 @@ void compute_kinetic(orc_ctx *o) {
 }
 
 // calculate_drift_without_noise (Propagator_without_cutoff.cu:1-129) plus the
-// spin term -c1 S_i (zero for c1 = 0) and the Q3 switch.
+// spin term -c1 S_i (zero for c1 = 0) and the conj-drift switch.
 void compute_drift(orc_ctx *o) {
   const clv_params &p = o->p;
   const int Nt = o->Nt;
@@ -273,8 +273,8 @@ void compute_drift(orc_ctx *o) {
         cd term_param = sfac[i] * pm[i];
         cd term_inter = -2.0 * p.G * tm - p.c1 * Sm[i];
         o->drift[idx(o, i, t, s)] = af * (pm[i] - p0[i] + p.eps * (term_param + term_inter + km));
-        // conjugate drift (:71-76); Q3: psi(t+1) in the reference mode
-        const cd xq = (p.q3_mode == CLV_Q3_PSIBAR) ? bp[i] : pp[i];
+        // conjugate drift (:71-76); psi(t+1) in the reference mode
+        const cd xq = (p.conj_drift_mode == CLV_CONJ_DRIFT_ACTION) ? bp[i] : pp[i];
         term_param = sfac[i] * xq;
         term_inter = -2.0 * p.G * tp - p.c1 * Sp[i];
         o->driftb[idx(o, i, t, s)] = af * (bp[i] - b0[i] + p.eps * (term_param + term_inter + kp));
@@ -283,7 +283,7 @@ void compute_drift(orc_ctx *o) {
 }
 
 // compute_max_and_mean_drift + get_max_and_mean_drift (:231-265, 351-380):
-// |D| over component 0 only, mean divided by 4*V_s*Nt (Q2).  Fixed order:
+// |D| over component 0 only, mean divided by 4*V_s*Nt (:353, 375).  Fixed order:
 // per-tau partials summed in tau order.
 void max_mean(const orc_ctx *o, double *mx, double *mean) {
   const int Nt = o->Nt;
@@ -405,7 +405,7 @@ void orc_set_params(orc_ctx *o, const clv_params *p) {
     for (int y = 0; y < o->Y; ++y)
       for (int z = 0; z < o->Z; ++z) {
         const double px = mom(x, o->X), py = mom(y, o->Y), pz = mom(z, o->Z);
-        // p2 = px*px (:196, Q1) unless the full Laplacian is asked for
+        // p2 = px*px (:196: p_x only) unless the full Laplacian is asked for
         const double p2 = (p->kin_dims == 3) ? px * px + py * py + pz * pz : px * px;
         o->p2[(size_t(x) * o->Y + y) * o->Z + z] = p2;
       }

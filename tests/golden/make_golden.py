@@ -16,16 +16,16 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
 
-from cl_var_dt_amd import CLV_Q3_PSIBAR, CLV_Q3_REFERENCE, Shape, default_params  # noqa: E402
+from cl_var_dt_amd import CLV_CONJ_DRIFT_ACTION, CLV_CONJ_DRIFT_REFERENCE, Shape, default_params  # noqa: E402
 from oracle import pyoracle  # noqa: E402
 
 CASES = {
     "traj_1d_16x8": dict(shape=(16, 1, 1, 8), steps=60, eval_every=5,
                          params=dict(mu=1.0, G=0.1, c1=-0.05, Q=0.1, P=0.02, eps=1 / 8, DT=1e-3,
-                                     seed=5, q3_mode=CLV_Q3_REFERENCE)),
+                                     seed=5, conj_drift_mode=CLV_CONJ_DRIFT_REFERENCE)),
     "traj_3d_8x4x2x4": dict(shape=(8, 4, 2, 4), steps=40, eval_every=4,
                             params=dict(mu=0.7, G=0.2, c1=0.05, Q=0.0, P=0.1, eps=0.25, DT=2e-3,
-                                        seed=17, q3_mode=CLV_Q3_PSIBAR)),
+                                        seed=17, conj_drift_mode=CLV_CONJ_DRIFT_ACTION)),
 }
 
 

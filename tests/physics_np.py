@@ -4,7 +4,7 @@ Written from the physics, not from the oracle's code: the kinetic operator via
 numpy.fft (pocketfft), the interaction as tr[(Psibar^T Psi)^2] with explicit
 2x2 matrices, the spin term with explicit Pauli matrices.  The complex-Langevin
 drift of psi is -dS/dpsibar, that of psibar is -dS/dpsi (reference drift
-Propagator_without_cutoff.cu:60-128 with a_x = factor = 1; its Q3 variant is
+Propagator_without_cutoff.cu:60-128 with a_x = factor = 1; its psi(t+1) variant is
 compared separately).
 """
 from __future__ import annotations

@@ -107,9 +107,9 @@ def test_driver_run_matches_oracle(tmp_path, oracle_lib):
 
 
 def test_integration_binding_compiles(tmp_path):
-    """The reference-side C++ binding shown in INTEGRATION.md (Option B: the
-    reference's Propagator members over clv.h) compiles and links against
-    libclv.so as written."""
+    """The documented reference-side C++ binding (the first ```cpp block of the
+    integration guide: the reference's Propagator members over clv.h) compiles
+    and links against libclv.so as written."""
     lib = os.path.join(ROOT, "cl_var_dt_amd", "libclv.so")
     if not os.path.exists(lib):
         pytest.skip("libclv.so not built")

@@ -11,7 +11,7 @@ observables (n_c, F_a(x), energy).
 import numpy as np
 import pytest
 
-from cl_var_dt_amd import CLV_FP32, CLV_FP64, CLV_Q3_PSIBAR, Shape, default_params
+from cl_var_dt_amd import CLV_FP32, CLV_FP64, CLV_CONJ_DRIFT_ACTION, Shape, default_params
 
 pytestmark = pytest.mark.gpu
 
@@ -109,7 +109,7 @@ def test_short_rows_fp32(oracle_lib, shape):
 def test_noise_free_update_matches(oracle_lib):
     print("case: noise-free update", flush=True)
     shape = (32, 8, 1, 16)
-    eng, orc, p = _pair(oracle_lib, shape, noise_on=0, factor=0.5, q3_mode=CLV_Q3_PSIBAR)
+    eng, orc, p = _pair(oracle_lib, shape, noise_on=0, factor=0.5, conj_drift_mode=CLV_CONJ_DRIFT_ACTION)
     rng = np.random.default_rng(4)
     d = eng.dims(0)
     f = 1 + 0.3 * (rng.normal(size=d) + 1j * rng.normal(size=d))

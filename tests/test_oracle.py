@@ -10,7 +10,7 @@ observables and hand values on constant fields.
 import numpy as np
 import pytest
 
-from cl_var_dt_amd import CLV_Q3_PSIBAR, CLV_Q3_REFERENCE, Shape, default_params
+from cl_var_dt_amd import CLV_CONJ_DRIFT_ACTION, CLV_CONJ_DRIFT_REFERENCE, Shape, default_params
 from tests import physics_np as ph
 
 
@@ -108,7 +108,7 @@ def test_kinetic_plane_waves(oracle_lib, dims):
         K, Kb = o.kinetic()
         assert np.allclose(K, -px ** 2 * f / 0.7, atol=1e-12)
         assert np.allclose(Kb, -px ** 2 * np.conj(f) / 0.7, atol=1e-12)
-    # Q1: a wave along y alone is annihilated (p^2 = p_x^2 only)
+    # the reference's p^2 = p_x^2: a wave along y alone is annihilated
     if Y > 1:
         wave = np.exp(2j * np.pi * y / Y) * np.ones((X, Y, Z))
         f = np.broadcast_to(wave, (4, Nt, X, Y, Z)).copy()
@@ -159,7 +159,7 @@ def _rand_fields(rng, sh, scale=0.6):
 @pytest.mark.parametrize("dims", [(4, 1, 1, 4), (4, 2, 1, 3)])
 def test_drift_is_minus_action_gradient(oracle_lib, c1, dims):
     X, Y, Z, Nt = dims
-    kw = dict(mu=0.8, P=0.13, Q=0.21, G=0.3, c1=c1, eps=0.07, a_x=1.0, q3_mode=CLV_Q3_PSIBAR)
+    kw = dict(mu=0.8, P=0.13, Q=0.21, G=0.3, c1=c1, eps=0.07, a_x=1.0, conj_drift_mode=CLV_CONJ_DRIFT_ACTION)
     o, p = _oracle(oracle_lib, dims, **kw)
     rng = np.random.default_rng(42)
     sh = (4, Nt, X, Y, Z)
@@ -209,12 +209,12 @@ def test_drift_invariant_form():
             assert abs(S_def - (N * q[i] - 2 * det_q * cof[i])) < 1e-12 * (1 + abs(S_def))
 
 
-def test_drift_q3_reference_mode(oracle_lib):
-    """Q3: in the reference the psibar drift's s_i term multiplies psi(t+1)."""
+def test_drift_conj_reference_mode(oracle_lib):
+    """In the reference the psibar drift's s_i term multiplies psi(t+1)."""
     dims = (8, 1, 1, 4)
     kw = dict(mu=0.6, P=0.2, Q=0.1, G=0.2, eps=0.05, a_x=1.3, factor=0.7)
-    o_ref, _ = _oracle(oracle_lib, dims, q3_mode=CLV_Q3_REFERENCE, **kw)
-    o_fix, _ = _oracle(oracle_lib, dims, q3_mode=CLV_Q3_PSIBAR, **kw)
+    o_ref, _ = _oracle(oracle_lib, dims, conj_drift_mode=CLV_CONJ_DRIFT_REFERENCE, **kw)
+    o_fix, _ = _oracle(oracle_lib, dims, conj_drift_mode=CLV_CONJ_DRIFT_ACTION, **kw)
     rng = np.random.default_rng(3)
     f, g = _rand_fields(rng, (4, 4, 8, 1, 1))
     for o in (o_ref, o_fix):
@@ -244,7 +244,7 @@ def test_drift_special_cases(oracle_lib):
     want = np.roll(f, 1, axis=1) * (1 + 0.1 * 0.9) - f
     assert np.allclose(D, want, atol=1e-14)
     # uniform fields psi = psibar = v: T_i = 4 v^3 (three n's plus the triple), D = eps(mu v - 8 G v^3)
-    o2, _ = _oracle(oracle_lib, dims, mu=0.5, G=0.2, eps=0.1, a_x=1.0, q3_mode=CLV_Q3_PSIBAR)
+    o2, _ = _oracle(oracle_lib, dims, mu=0.5, G=0.2, eps=0.1, a_x=1.0, conj_drift_mode=CLV_CONJ_DRIFT_ACTION)
     v = 0.9 + 0.2j
     f = np.full(sh, v)
     o2.set_fields(f, f)
@@ -279,7 +279,7 @@ def test_max_mean_is_component0_over_4S(oracle_lib):
     mx, mean = o.max_mean()
     a0 = np.abs(D[0])
     assert mx == pytest.approx(a0.max(), rel=1e-15)
-    assert mean == pytest.approx(a0.sum() / D.size, rel=1e-13)   # Q2: / (4 V_s Nt)
+    assert mean == pytest.approx(a0.sum() / D.size, rel=1e-13)   # / (4 V_s Nt)
 
 
 # ---------------------------------------------------------------- update ----
@@ -297,7 +297,7 @@ def test_update_noise_conjugation_and_stream(oracle_lib):
     f1, g1 = o.get_fields()
     eta = f1 - f - dt * D
     etab = g1 - g - dt * Db
-    assert np.allclose(etab, np.conj(eta), atol=1e-14)        # Q5
+    assert np.allclose(etab, np.conj(eta), atol=1e-14)        # conj(eta) in psibar
     # eta at (c, t, x, y, z) = sqrt(dt) * normal pair of (seed, 0, step 0, site, c // 2)
     for (c, t, x, y, z) in [(0, 0, 0, 0, 0), (1, 3, 7, 1, 0), (2, 1, 4, 0, 0), (3, 2, 5, 1, 0)]:
         site = ((t * 8 + x) * 2 + y) * 1 + z

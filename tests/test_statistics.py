@@ -15,7 +15,7 @@
 import numpy as np
 import pytest
 
-from cl_var_dt_amd import CLV_FP32, CLV_FP64, CLV_Q3_PSIBAR, Shape, default_params
+from cl_var_dt_amd import CLV_FP32, CLV_FP64, CLV_CONJ_DRIFT_ACTION, Shape, default_params
 
 pytestmark = pytest.mark.gpu
 
@@ -33,7 +33,7 @@ def test_free_theory_occupation():
     eng = _engine([shape] * nch, CLV_FP32)
     for i in range(nch):
         eng.set_params(i, default_params(mu=mu, G=0.0, c1=0.0, P=0.0, Q=0.0, eps=eps, DT=1e-2,
-                                         q3_mode=CLV_Q3_PSIBAR, seed=3000 + i))
+                                         conj_drift_mode=CLV_CONJ_DRIFT_ACTION, seed=3000 + i))
         eng.init_fields(i, 0.0, 0.5)
     eng.propagate(20000, 0)          # thermalise (slowest mode relaxes in ~20 time units)
     eng.reset_observables()
