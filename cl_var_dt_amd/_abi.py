@@ -164,6 +164,7 @@ def _declare(lib):
         "clv_server_submit": (C.c_int, [vp, P(Request), P(i64)]),
         "clv_server_poll": (C.c_int, [vp, i64, P(Result)]),
         "clv_server_wait": (C.c_int, [vp, i64, dbl, P(Result)]),
+        "clv_server_release": (C.c_int, [vp, i64]),
         "clv_server_info_get": (C.c_int, [vp, P(ServerInfo)]),
         "clv_server_clock": (dbl, []),
         "clv_server_error": (C.c_char_p, [vp]),
@@ -186,7 +187,7 @@ def exported_symbols() -> list[str]:
         "clv_reset_observables", "clv_autotune", "clv_set_schedule", "clv_get_schedule",
         "clv_set_fused_options", "clv_set_profiling", "clv_kernel_time", "clv_set_graphs",
         "clv_server_create", "clv_server_destroy", "clv_server_submit", "clv_server_poll",
-        "clv_server_wait", "clv_server_info_get", "clv_server_clock", "clv_server_error",
+        "clv_server_wait", "clv_server_release", "clv_server_info_get", "clv_server_clock", "clv_server_error",
     ]
 
 

@@ -74,7 +74,7 @@ struct clv_server {
   int device = 0;
   int precision = CLV_FP32;
   int64_t budget = 0;
-  std::mutex mu;
+  mutable std::mutex mu;
   std::condition_variable cv_work, cv_done;
   std::deque<Run *> queue;
   std::map<int64_t, std::unique_ptr<Run>> runs;  // guarded by mu
@@ -346,7 +346,15 @@ extern "C" {
 
 double clv_server_clock(void) { return now_s(); }
 
-const char *clv_server_error(const clv_server *srv) { return srv ? srv->err.c_str() : ""; }
+// The worker may set the message at any time: the caller gets a copy, valid
+// until this thread's next call.
+const char *clv_server_error(const clv_server *srv) {
+  static thread_local std::string copy;
+  if (!srv) return "";
+  std::lock_guard<std::mutex> g(srv->mu);
+  copy = srv->err;
+  return copy.c_str();
+}
 
 int clv_server_create(int32_t device, int32_t precision, int64_t hbm_budget_bytes,
                       clv_server **out) {
@@ -437,6 +445,20 @@ int clv_server_wait(clv_server *s, int64_t id, double timeout_s, clv_result *out
   });
   *out = r->res;
   return ok ? CLV_OK : CLV_ERR_STATE;
+}
+
+int clv_server_release(clv_server *s, int64_t id) {
+  if (!s) return CLV_ERR_ARG;
+  std::lock_guard<std::mutex> g(s->mu);
+  auto it = s->runs.find(id);
+  if (it == s->runs.end()) return CLV_ERR_NOTFOUND;
+  const int st = it->second->res.status;
+  if (st == CLV_RUN_QUEUED || st == CLV_RUN_RUNNING) {
+    s->err = "clv_server_release: run " + std::to_string(id) + " is not finished";
+    return CLV_ERR_STATE;
+  }
+  s->runs.erase(it);
+  return CLV_OK;
 }
 
 int clv_server_info_get(clv_server *s, clv_server_info *out) {

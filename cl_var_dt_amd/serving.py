@@ -76,11 +76,20 @@ class Server:
         _abi.check(lib().clv_server_poll(self._h, rid, C.byref(r)))
         return r
 
-    def wait(self, rid: int, timeout: float = 600.0) -> RunResult:
+    def release(self, rid: int) -> None:
+        """Forget a finished run's record (clv_server_release)."""
+        st = lib().clv_server_release(self._h, rid)
+        if st != _abi.CLV_OK:
+            raise _abi.ClvError(st, lib().clv_server_error(self._h).decode())
+
+    def wait(self, rid: int, timeout: float = 600.0, release: bool = False) -> RunResult:
+        """Block until run `rid` finishes; release=True also forgets its record."""
         r = Result()
         st = lib().clv_server_wait(self._h, rid, timeout, C.byref(r))
         if st != _abi.CLV_OK:
             raise _abi.ClvError(st, f"run {rid} not done after {timeout} s")
+        if release:
+            self.release(rid)
         if r.status < 0:
             raise _abi.ClvError(r.status, lib().clv_server_error(self._h).decode())
         return RunResult(id=r.id, density=r.obs.density, n_tot=r.obs.n_tot, Sz=r.obs.Sz,

@@ -262,9 +262,13 @@ int clv_server_submit(clv_server *srv, const clv_request *req, int64_t *id);
 int clv_server_poll(clv_server *srv, int64_t id, clv_result *out);
 /* Blocks until the run is done or timeout_s passes (CLV_ERR_STATE). */
 int clv_server_wait(clv_server *srv, int64_t id, double timeout_s, clv_result *out);
+/* Forgets a finished (done or failed) run's record, which the server keeps
+ * for poll/wait until then; CLV_ERR_STATE while it is queued or running. */
+int clv_server_release(clv_server *srv, int64_t id);
 int clv_server_info_get(clv_server *srv, clv_server_info *out);
 /* The server's monotonic clock, seconds. */
 double clv_server_clock(void);
+/* Last error message; a copy valid until the calling thread's next call. */
 const char *clv_server_error(const clv_server *srv);
 
 #ifdef __cplusplus

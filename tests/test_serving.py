@@ -117,3 +117,26 @@ def test_secant_mu_search_through_the_server():
     assert abs(hist[-1][1]) < 0.05 or len(hist) == 10
     # the search moved toward the target
     assert abs(hist[-1][1]) < abs(hist[0][1])
+
+
+@pytest.mark.gpu
+def test_release_forgets_finished_runs_only():
+    print("case: serving release", flush=True)
+    from cl_var_dt_amd import _abi
+    shape = Shape(16, 16, 1, 8)
+    with serving.Server(precision=CLV_FP32) as srv:
+        rid = srv.submit(shape, _p(5), nsteps=2000)
+        with pytest.raises(_abi.ClvError) as e:       # queued or running: kept
+            srv.release(rid)
+        assert e.value.status == _abi.CLV_ERR_STATE
+        assert srv.wait(rid, timeout=120).steps == 2000
+        srv.release(rid)
+        with pytest.raises(_abi.ClvError) as e:
+            srv.poll(rid)
+        assert e.value.status == _abi.CLV_ERR_NOTFOUND
+        with pytest.raises(_abi.ClvError):
+            srv.release(rid)
+        rid2 = srv.submit(shape, _p(6), nsteps=20)
+        assert srv.wait(rid2, timeout=60, release=True).steps == 20
+        with pytest.raises(_abi.ClvError):
+            srv.poll(rid2)
