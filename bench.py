@@ -242,7 +242,8 @@ def run_serving(args, log):
 
 
 def run_cpu_baseline(args, log):
-    """FP64 oracle (C++/OpenMP) on the plain workload's shape, time-bounded sample."""
+    """FP64 oracle (C++/OpenMP), time-bounded sample of the batched workload:
+    its chain 0 (same lattice, parameters and seed), observables every 10."""
     from cl_var_dt_amd import Shape
     from oracle import pyoracle
 
@@ -250,8 +251,9 @@ def run_cpu_baseline(args, log):
         ncores = len(os.sched_getaffinity(0))
     except AttributeError:  # pragma: no cover
         ncores = os.cpu_count() or 1
-    shape = Shape(64, 64, 1, 64)
-    orc = pyoracle.Oracle(shape, _params(7, nt=64), nthreads=ncores)
+    X, Nt = args.lx, args.nt
+    shape = Shape(X, X, 1, Nt)
+    orc = pyoracle.Oracle(shape, _params(1000, mu=0.5, nt=Nt), nthreads=ncores)
     orc.init_fields(1.0, 0.1)
     orc.propagate(2, 0)
     steps, t0 = 0, time.perf_counter()
@@ -262,11 +264,12 @@ def run_cpu_baseline(args, log):
         if el >= args.cpu_seconds or steps >= 2000:
             break
     orc.close()
-    sites = 64 * 64 * 64
+    sites = X * X * Nt
     log(f"cpu baseline: {steps} steps in {el:.2f} s on {ncores} threads")
     return {"value": sites * steps / el, "unit": "site-updates/s", "cores": ncores, "kind": "port",
-            "sample": f"FP64 oracle (project Stockham FFT, OpenMP), 2-D 64x64x1x64 chain, "
-                      f"{steps} adaptive steps with observables every 10, {el:.1f} s"}
+            "sample": f"FP64 oracle (project Stockham FFT, OpenMP), chain 0 of the batched workload "
+                      f"(2-D {X}x{X}x1x{Nt}, mu=0.5), {steps} adaptive steps with observables every 10, "
+                      f"{el:.1f} s"}
 
 
 def main() -> int:
