@@ -345,6 +345,23 @@ def test_init_fields(oracle_lib):
 
 
 # ----------------------------------------------------------- observables ----
+def test_spin_densities_are_pauli_bilinears(oracle_lib):
+    """F_a(x) = sum_t Re tr(Psibar(t+1)^T sigma_a Psi(t)) / Nt with explicit Pauli
+    matrices, site by site on random fields (evaluate_spin_density_realspace,
+    Observables_without_cutoff.cu:456-488, pairs psibar at m+1 with psi at m)."""
+    X, Y, Z, Nt = 6, 2, 3, 4
+    o, p = _oracle(oracle_lib, (X, Y, Z, Nt), mu=0.5, G=0.3, c1=0.05, a_x=1.0)
+    rng = np.random.default_rng(23)
+    f, g = _rand_fields(rng, (4, Nt, X, Y, Z))
+    o.set_fields(f, g)
+    o.reset_observables()
+    o.evaluate()
+    obs, F = o.observables()
+    assert obs.count == 1
+    B, Q = ph.as_matrix(np.roll(g, -1, axis=1)), ph.as_matrix(f)
+    for a, sig in enumerate(ph.PAULI):
+        want = np.einsum("rct...,rs,sct...->t...", B, sig, Q).real.sum(axis=0) / Nt
+        assert np.allclose(F[a], want.reshape(-1), rtol=1e-12, atol=1e-12), a
 def test_observables_parseval_and_hand_values(oracle_lib):
     dims = (8, 2, 1, 4)
     sh = (4, 4, 8, 2, 1)
