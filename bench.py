@@ -285,8 +285,8 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--quiet", action="store_true")
-    ap.add_argument("--mode", choices=["all", "batched", "copy"], default="all",
-                    help="batched: only the batched leg (for profiling runs); copy: a 1 GiB "
+    ap.add_argument("--mode", choices=["all", "batched", "uneven", "copy"], default="all",
+                    help="batched / uneven: only that leg (for profiling runs); copy: a 1 GiB "
                          "device-to-device copy, the calibration kernel of the traffic counters")
     ap.add_argument("--uneven-chains", type=int, default=256)
     ap.add_argument("--uneven-steps", type=int, default=20)
@@ -323,6 +323,9 @@ def main() -> int:
             y.copy_(x)
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes": x.numel() * 4, "copies": 3}), flush=True)
+        return 0
+    if args.mode == "uneven":
+        print(json.dumps(run_uneven(args, log)), flush=True)
         return 0
     b = run_batched(args, log)
     if args.mode == "batched":
