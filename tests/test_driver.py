@@ -106,6 +106,53 @@ def test_driver_run_matches_oracle(tmp_path, oracle_lib):
     assert np.allclose(got, np.array(rows), atol=2e-6)
 
 
+def _ext_params_h(dt_device):
+    """A 3-D lattice with the engine's Parameters.h extensions."""
+    h = PARAMS_H.replace("#define XSIZE 32\n#define YSIZE 1\n#define ZSIZE 1\n",
+                         "#define XSIZE 16\n#define YSIZE 4\n#define ZSIZE 4\n")
+    ext = ("#define C1 0.05\n#define KIN_DIMS 3\n#define CONJ_DRIFT_MODE 1\n#define INIT gaussian\n"
+           "#define INIT_N0 1.0\n#define INIT_VAR 0.05\n#define DT_DEVICE %d\n#endif\n" % dt_device)
+    return h.replace("#endif\n", ext)
+
+
+@pytest.mark.gpu
+def test_driver_extensions_match_oracle(tmp_path, oracle_lib):
+    """The Parameters.h extensions (C1, KIN_DIMS = 3, CONJ_DRIFT_MODE = 1, a
+    Gaussian start) on a 16x4x4x8 lattice: host-driven and device-resident dt
+    write identical files, and Skalars.txt equals the oracle run through the
+    same main loop (main.cu:144-158)."""
+    print("case: driver extensions", flush=True)
+    outs = []
+    for dtd in (0, 1):
+        d = tmp_path / ("dt%d" % dtd)
+        d.mkdir()
+        r = _run(d, "0", params_h=_ext_params_h(dtd))
+        assert r.returncode == 0, r.stdout + r.stderr
+        base = d / "DATA" / "BoseCL_Sim_mu_-0.5_P_0"
+        outs.append([(base / "Skalars.txt").read_text()] +
+                    [(base / "Run1" / ("F_%s.txt" % a)).read_text() for a in "xyz"])
+    assert outs[0] == outs[1]
+    from cl_var_dt_amd import CLV_CONJ_DRIFT_ACTION, Shape, default_params
+    p = default_params(mu=-0.5, P=0.0, G=0.1, Q=0.0, c1=0.05, eps=0.125, a_x=1.0, DT=0.002, seed=42,
+                       kin_dims=3, conj_drift_mode=CLV_CONJ_DRIFT_ACTION)
+    o = oracle_lib.Oracle(Shape(16, 4, 4, 8), p, nthreads=2)
+    o.init_fields(1.0, 0.05)
+    rows = []
+    for i in range(120):
+        if (i % 50 == 0 and i not in (0, 119)) or i == 1:
+            obs, _ = o.observables()
+            rows.append([obs.n_tot, *obs.n_comp])
+        o.propagate(1, 0)
+        if i % 10 == 0:
+            o.evaluate()
+    obs, _ = o.observables()
+    rows.append([obs.n_tot, *obs.n_comp])
+    sk = [ln.split("\t") for ln in outs[0][0].splitlines()]
+    got = np.array([[float(v) for v in row[:5]] for row in sk])
+    assert got.shape == (4, 5) and np.all(np.isfinite(got))
+    assert np.allclose(got, np.array(rows), atol=2e-6)
+
+
 def test_integration_binding_compiles(tmp_path):
     """The documented reference-side C++ binding (the first ```cpp block of the
     integration guide: the reference's Propagator members over clv.h) compiles

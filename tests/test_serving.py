@@ -12,7 +12,7 @@ import time
 import numpy as np
 import pytest
 
-from cl_var_dt_amd import CLV_FP32, Shape, default_params
+from cl_var_dt_amd import CLV_FP32, CLV_FP64, Shape, default_params
 from cl_var_dt_amd import serving
 
 
@@ -67,14 +67,15 @@ def test_same_result_alone_and_in_the_mix():
 
 
 @pytest.mark.gpu
-def test_server_run_equals_plain_context():
-    print("case: serving vs context", flush=True)
+@pytest.mark.parametrize("prec", [CLV_FP32, CLV_FP64])
+def test_server_run_equals_plain_context(prec):
+    print("case: serving vs context", prec, flush=True)
     from cl_var_dt_amd.engine import Engine
     shape = Shape(32, 32, 1, 16)
     p = _p(5, mu=0.8, nt=16)
-    with serving.Server(precision=CLV_FP32) as srv:
+    with serving.Server(precision=prec) as srv:
         r = srv.wait(srv.submit(shape, p, nsteps=75, eval_every=5, eval_from=20))
-    eng = Engine([shape], precision=CLV_FP32)
+    eng = Engine([shape], precision=prec)
     eng.set_params(0, p)
     eng.init_fields(0, 1.0, 0.1)
     for i in range(75):
