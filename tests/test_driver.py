@@ -153,6 +153,50 @@ def test_driver_extensions_match_oracle(tmp_path, oracle_lib):
     assert np.allclose(got, np.array(rows), atol=2e-6)
 
 
+def _oracle_main_loop(o, steps=120):
+    """Skalars rows (n_tot, n_0..n_3) of main.cu's loop (:144-158) on the oracle."""
+    rows = []
+    for i in range(steps):
+        if (i % 50 == 0 and i not in (0, steps - 1)) or i == 1:
+            obs, _ = o.observables()
+            rows.append([obs.n_tot, *obs.n_comp])
+        o.propagate(1, 0)
+        if i % 10 == 0:
+            o.evaluate()
+    obs, _ = o.observables()
+    rows.append([obs.n_tot, *obs.n_comp])
+    return np.array(rows)
+
+
+@pytest.mark.gpu
+def test_driver_mean_field_start_and_fp32(tmp_path, oracle_lib):
+    """INIT mean_field (set_value, Lattice.cu:50-81: sqrt(|mu|/G/4) in components
+    0 and 3, 2/3 of it in 1 and 2) against the oracle from the same fields, and
+    the same run with FP32 fields (PRECISION 1) within FP32 rounding."""
+    print("case: driver mean field + fp32", flush=True)
+    got = []
+    for prec in (0, 1):
+        d = tmp_path / ("p%d" % prec)
+        d.mkdir()
+        h = PARAMS_H.replace("#endif\n", "#define INIT mean_field\n#define PRECISION %d\n#endif\n" % prec)
+        r = _run(d, "0", params_h=h)
+        assert r.returncode == 0, r.stdout + r.stderr
+        sk = (d / "DATA" / "BoseCL_Sim_mu_-0.5_P_0" / "Skalars.txt").read_text().splitlines()
+        got.append(np.array([[float(v) for v in ln.split("\t")[:5]] for ln in sk]))
+    from cl_var_dt_amd import Shape, default_params
+    p = default_params(mu=-0.5, P=0.0, G=0.1, Q=0.0, eps=0.125, a_x=1.0, DT=0.002, seed=42)
+    o = oracle_lib.Oracle(Shape(32, 1, 1, 8), p, nthreads=2)
+    amp = np.sqrt(0.5 / 0.1 / 4)
+    f = np.zeros((4, 8, 32, 1, 1), complex)
+    for c, w in enumerate((1.0, 2.0 / 3.0, 2.0 / 3.0, 1.0)):
+        f[c] = w * amp
+    o.set_fields(f, np.conj(f))
+    want = _oracle_main_loop(o)
+    assert got[0].shape == want.shape == (4, 5)
+    assert np.allclose(got[0], want, atol=2e-6)
+    assert np.allclose(got[1], got[0], rtol=1e-4)
+
+
 def test_integration_binding_compiles(tmp_path):
     """The documented reference-side C++ binding (the first ```cpp block of the
     integration guide: the reference's Propagator members over clv.h) compiles
