@@ -336,19 +336,11 @@ def test_fused_schedule_bit_identical_to_two_pass(prec):
         e.close()
 
 
-@pytest.mark.parametrize("prec", [CLV_FP32, CLV_FP64])
-def test_fused_options_bit_identical_to_two_pass(prec):
-    """Every fused option (LDS line sets 1/2, next-step K_0 lines transformed
-    1/2/4 taus at a time) reproduces the two-pass schedule bit for bit: Nt from
-    2 to 9 and 32 cover batches that end on, before and after the last tau,
-    the parked taus flushed with tau = 0, and a first batch without tau = 0."""
-    print("case: fused options", prec, flush=True)
+def _fused_options_match_two_pass(shapes, prec):
+    """Run `shapes` with every fused option that fits and compare each with the
+    two-pass schedule bit for bit; returns the number of options tried."""
     from cl_var_dt_amd import ClvError
     from cl_var_dt_amd._abi import SCHED_FUSED, SCHED_TWO_PASS
-    shapes = [(128, 2, 1, 32), (128, 2, 1, 5), (64, 4, 1, 7), (16, 8, 2, 9), (24, 4, 1, 6),
-              (16, 4, 1, 2), (8, 2, 1, 3), (32, 2, 1, 4), (128, 128, 1, 8)]
-    if prec == CLV_FP64:
-        shapes = [(32, 2, 1, 8), (16, 4, 1, 6), (16, 4, 1, 5), (8, 2, 1, 3), (8, 4, 1, 7)]
 
     def run(sched, opts=None):
         e = _engine(shapes, prec)
@@ -385,7 +377,31 @@ def test_fused_options_bit_identical_to_two_pass(prec):
             assert oa.n_tot == ob.n_tot and oa.energy == ob.energy, (opts, shapes[i])
         e.close()
     ref.close()
-    assert tried >= (4 if prec == CLV_FP32 else 1)
+    return tried
+
+
+@pytest.mark.parametrize("prec", [CLV_FP32, CLV_FP64])
+def test_fused_options_bit_identical_to_two_pass(prec):
+    """Every fused option (LDS line sets 1/2, next-step K_0 lines transformed
+    1/2/4 taus at a time) reproduces the two-pass schedule bit for bit: Nt from
+    2 to 9 and 32 cover batches that end on, before and after the last tau,
+    the parked taus flushed with tau = 0, and a first batch without tau = 0;
+    row lengths 4 to 256 cover every split of the row transform.  Shapes whose
+    LDS admits fewer options run as a group of their own."""
+    print("case: fused options", prec, flush=True)
+    if prec == CLV_FP32:
+        groups = [[(128, 2, 1, 32), (128, 2, 1, 5), (64, 4, 1, 7), (16, 8, 2, 9), (24, 4, 1, 6),
+                   (16, 4, 1, 2), (8, 2, 1, 3), (32, 2, 1, 4), (128, 128, 1, 8),
+                   (96, 2, 1, 4), (48, 4, 1, 3), (4, 8, 2, 3)],
+                  [(256, 2, 1, 6), (192, 2, 1, 5)]]
+        floors = [4, 4]
+    else:
+        groups = [[(32, 2, 1, 8), (16, 4, 1, 6), (16, 4, 1, 5), (8, 2, 1, 3), (8, 4, 1, 7),
+                   (48, 2, 1, 3)],
+                  [(256, 1, 1, 3), (96, 2, 1, 4)]]
+        floors = [4, 2]
+    for shapes, floor in zip(groups, floors):
+        assert _fused_options_match_two_pass(shapes, prec) >= floor, shapes
 
 
 @pytest.mark.parametrize("prec", [CLV_FP32, CLV_FP64])
