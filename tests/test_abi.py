@@ -90,3 +90,11 @@ def test_create_rejects_bad_shapes(libclv):
         h = C.c_void_p()
         st = libclv.clv_create(shp, 1, _abi.CLV_FP64, 0, C.byref(h))
         assert st == _abi.CLV_ERR_ARG, bad
+    # 2^31 sites per chain: the 32-bit site counter of the noise would wrap
+    shp = (_abi.Shape * 1)(_abi.Shape(256, 1024, 1024, 8))
+    h = C.c_void_p()
+    assert libclv.clv_create(shp, 1, _abi.CLV_FP32, 0, C.byref(h)) == _abi.CLV_ERR_ARG
+    assert b"too large" in libclv.clv_last_error(None)
+    shp = (_abi.Shape * 1)(_abi.Shape(8, 1, 1, 4))
+    assert libclv.clv_create(shp, 1, 7, 0, C.byref(h)) == _abi.CLV_ERR_ARG
+    assert b"precision" in libclv.clv_last_error(None)
