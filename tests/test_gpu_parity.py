@@ -608,3 +608,47 @@ def test_autotune_keeps_state_and_caches(tmp_path):
     assert len(open(cache).read().strip().splitlines()) == 2
     for eng in (e, ref, e2):
         eng.close()
+
+
+def test_full_size_batch_properties():
+    """The batched workload at its full size (128 chains of 128x128x1x32,
+    FP32, mu on a grid in [0.5, 1.5], observables every 10 steps):
+    fused graph-replayed steps equal two-pass eager steps bit for bit, and a
+    chain run alone equals the same chain in the batch.  Both properties are
+    independent of size, so they carry the small-shape parity with the oracle
+    to this size."""
+    print("case: full-size batch properties", flush=True)
+    from cl_var_dt_amd._abi import SCHED_FUSED, SCHED_TWO_PASS
+    nch, shape = 128, (128, 128, 1, 32)
+    probe = [0, 63, 127]
+
+    def batch(sched, graphs):
+        e = _engine([shape] * nch, CLV_FP32)
+        for i in range(nch):
+            e.set_params(i, _params(seed=1000 + i, mu=0.5 + i / (nch - 1), c1=-0.05, P=0.0))
+            e.init_fields(i, 1.0, 0.1)
+        e.set_schedule(sched)
+        e.set_graphs(graphs)
+        e.propagate(12, 10)
+        e.synchronize()
+        out = {i: (e.get_fields(i), e.stats(i), e.observables(i)[0]) for i in probe}
+        e.close()
+        return out
+
+    fused = batch(SCHED_FUSED, True)
+    two = batch(SCHED_TWO_PASS, False)
+    alone = _engine([shape], CLV_FP32)
+    alone.set_params(0, _params(seed=1000 + 127, mu=1.5, c1=-0.05, P=0.0))
+    alone.init_fields(0, 1.0, 0.1)
+    alone.propagate(12, 10)
+    a_f, a_s, a_o = alone.get_fields(0), alone.stats(0), alone.observables(0)[0]
+    alone.close()
+    for i in probe:
+        (ff, fs, fo), (tf, ts, to) = fused[i], two[i]
+        assert np.array_equal(ff[0], tf[0]) and np.array_equal(ff[1], tf[1]), i
+        assert (fs.dt, fs.t_langevin, fs.steps) == (ts.dt, ts.t_langevin, ts.steps), i
+        assert fo.count == to.count == 2 and fo.n_tot == to.n_tot and fo.energy == to.energy, i
+        assert np.all(np.isfinite(ff[0])) and fs.nonfinite == 0, i
+    ff, fs, fo = fused[127]
+    assert np.array_equal(ff[0], a_f[0]) and np.array_equal(ff[1], a_f[1])
+    assert (fs.dt, fs.t_langevin) == (a_s.dt, a_s.t_langevin) and fo.n_tot == a_o.n_tot
