@@ -652,3 +652,53 @@ def test_full_size_batch_properties():
     ff, fs, fo = fused[127]
     assert np.array_equal(ff[0], a_f[0]) and np.array_equal(ff[1], a_f[1])
     assert (fs.dt, fs.t_langevin) == (a_s.dt, a_s.t_langevin) and fo.n_tot == a_o.n_tot
+
+
+def test_full_size_uneven_batch_properties():
+    """The uneven batch at its full size (256 chains of L^3 x Nt, L in {16, 24,
+    32}, Nt in {16, 32, 64}, drawn from a seed, beta in [0.5, 2], FP32):
+    graph-captured steps with the autotuned per-bucket schedules equal those
+    with the default schedules bit for bit, and one chain of each L run alone
+    equals the same chain in the mix."""
+    print("case: full-size uneven batch properties", flush=True)
+    import random
+    rng = random.Random(3)
+    shapes, betas = [], []
+    for _ in range(256):
+        L = rng.choice([16, 24, 32])
+        nt = rng.choice([16, 32, 64])
+        shapes.append((L, L, L, nt))
+        betas.append(rng.uniform(0.5, 2.0))
+    probe = [next(i for i, s in enumerate(shapes) if s[0] == L) for L in (16, 24, 32)]
+
+    def params(i):
+        nt = shapes[i][3]
+        return _params(seed=5000 + i, mu=1.0, eps=betas[i] / nt, c1=-0.05, P=0.0)
+
+    def batch(tune):
+        e = _engine(shapes, CLV_FP32)
+        for i in range(len(shapes)):
+            e.set_params(i, params(i))
+            e.init_fields(i, 1.0, 0.1)
+        if tune:
+            e.autotune(None)
+        e.propagate(12, 10)
+        e.synchronize()
+        out = {i: (e.get_fields(i), e.stats(i), e.observables(i)[0]) for i in probe}
+        e.close()
+        return out
+
+    base, tuned = batch(False), batch(True)
+    for i in probe:
+        alone = _engine([shapes[i]], CLV_FP32)
+        alone.set_params(0, params(i))
+        alone.init_fields(0, 1.0, 0.1)
+        alone.propagate(12, 10)
+        a = (alone.get_fields(0), alone.stats(0), alone.observables(0)[0])
+        alone.close()
+        for other in (tuned[i], a):
+            (bf, bs, bo), (of, os_, oo) = base[i], other
+            assert np.array_equal(bf[0], of[0]) and np.array_equal(bf[1], of[1]), (i, shapes[i])
+            assert (bs.dt, bs.t_langevin, bs.steps) == (os_.dt, os_.t_langevin, os_.steps), i
+            assert bo.count == oo.count == 2 and bo.n_tot == oo.n_tot and bo.energy == oo.energy, i
+        assert base[i][1].nonfinite == 0
