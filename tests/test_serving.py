@@ -141,3 +141,27 @@ def test_release_forgets_finished_runs_only():
         assert srv.wait(rid2, timeout=60, release=True).steps == 20
         with pytest.raises(_abi.ClvError):
             srv.poll(rid2)
+
+
+@pytest.mark.gpu
+def test_largest_request_same_alone_and_in_the_mix():
+    """A run of the largest serving shape (256x256x1x128, 8.4 M sites) gives the
+    same result alone and admitted while other shapes are in flight."""
+    print("case: serving largest request", flush=True)
+    big = Shape(256, 256, 1, 128)
+    pb = _p(31, mu=1.1, nt=128)
+    with serving.Server(precision=CLV_FP32) as srv:
+        alone = srv.wait(srv.submit(big, pb, nsteps=30, eval_every=10), release=True)
+    with serving.Server(precision=CLV_FP32) as srv:
+        ids = [srv.submit(Shape(128, 128, 1, 64), _p(32, mu=0.7, nt=64), nsteps=60, eval_every=10),
+               srv.submit(Shape(64, 64, 1, 32), _p(33, mu=1.3, nt=32), nsteps=80, eval_every=10)]
+        time.sleep(0.02)
+        rid = srv.submit(big, pb, nsteps=30, eval_every=10)
+        ids.append(srv.submit(Shape(256, 256, 1, 32), _p(34, mu=0.9, nt=32), nsteps=40, eval_every=10))
+        mixed = srv.wait(rid, release=True)
+        for i in ids:
+            srv.wait(i, release=True)
+        assert srv.info().failed == 0
+    assert mixed.count == alone.count == 3
+    for f in ("n_tot", "density", "Sz", "energy", "t_langevin", "steps"):
+        assert getattr(mixed, f) == getattr(alone, f), f
