@@ -5,7 +5,8 @@ mu search of the reference's orchestration as a client of it.
 admitted into device-resident slot pools by HBM budget while others are in
 flight, and return their observables and time-to-result.  `secant_mu_search`
 restates the secant update of mylib.py:561-584 (new_mu / check_to_stop) with
-follow-up runs issued to the server toward a target density.
+follow-up runs issued to the server toward a target density; `windowed_means`
+restates the burn-in-cut running means of mylib.py:384-404.
 """
 from __future__ import annotations
 
@@ -116,6 +117,39 @@ def new_mu(r: Sequence[float], mu: Sequence[float]) -> float:
 def check_to_stop(residual: float, err: float) -> bool:
     """mylib.py:582-584: stop once |observable - target| < err."""
     return abs(residual) < err
+
+
+def windowed_means(series, cut_fraction: float = 0.0, excluded: Sequence[int] = ()):
+    """Running means after a burn-in cut, across runs, from the cumulative means
+    a run writes at each snapshot (one column of Skalars.txt per run): the
+    arithmetic of compute_rhos (mylib.py:384-404).
+
+    series: (runs, rows) cumulative means; i0 = int(cut_fraction * rows).  For
+    i = 1 .. rows - i0 - 1 and each run r not in `excluded` (1-based run
+    numbers, as the reference's),
+        v_r(i) = (i0 + i + 1) / (i + 1) * (m_r[i0 + i] - i0 / (i0 + i + 1) * m_r[i0]),
+    the mean over rows i0 .. i0 + i with row k weighted as k + 1 snapshots and
+    the start row as i0 (the reference's weights, kept as written).  Returns
+    (evol, evol_err): the mean of v_r(i) over runs and its population standard
+    deviation / sqrt(runs - len(excluded)).  A NaN in m_r[i0 + i] raises
+    ValueError, as the reference raises NanError.  With runs windowed by
+    eval_from (clv_request) the server returns such window means directly.
+    """
+    import numpy as np
+
+    m = np.asarray(series, dtype=float)
+    runs, rows = m.shape
+    keep = [r for r in range(runs) if r + 1 not in excluded]
+    i0 = int(cut_fraction * rows)
+    evol, err = [], []
+    for i in range(1, rows - i0):
+        last = m[keep, i0 + i]
+        if np.isnan(last).any():
+            raise ValueError("NaN at line %d" % (i0 + i))
+        v = (i0 + (i + 1)) / (i + 1) * (last - i0 / (i0 + (i + 1)) * m[keep, i0])
+        evol.append(float(np.mean(v)))
+        err.append(float(np.var(v) ** 0.5 / (runs - len(excluded)) ** 0.5))
+    return np.array(evol), np.array(err)
 
 
 def secant_mu_search(run_density: Callable[[float], float], mu0: float, mu1: float,

@@ -165,3 +165,22 @@ def test_largest_request_same_alone_and_in_the_mix():
     assert mixed.count == alone.count == 3
     for f in ("n_tot", "density", "Sz", "energy", "t_langevin", "steps"):
         assert getattr(mixed, f) == getattr(alone, f), f
+
+
+def test_windowed_means_closed_forms():
+    # no cut: the window means are the cumulative means themselves
+    m = np.array([[1.0, 2.0, 3.0, 4.0], [3.0, 2.0, 5.0, 0.0]])
+    evol, err = serving.windowed_means(m, 0.0)
+    assert np.allclose(evol, m[:, 1:].mean(axis=0)) and np.allclose(err, m[:, 1:].std(axis=0) / 2 ** 0.5)
+    # constant series: the window mean is the constant, with zero spread
+    evol, err = serving.windowed_means(np.full((3, 10), 0.7), 0.4)
+    assert np.allclose(evol, 0.7) and np.allclose(err, 0.0) and len(evol) == 10 - 4 - 1
+    # by hand, rows = 4, cut 0.5: i0 = 2, i = 1: v = (4/2) (m3 - (2/4) m2) = 2 m3 - m2
+    m = np.array([[0.0, 0.0, 1.0, 2.0], [0.0, 0.0, 3.0, 2.0]])
+    evol, err = serving.windowed_means(m, 0.5)
+    assert np.allclose(evol, [2.0]) and np.allclose(err, [1.0 / 2 ** 0.5])
+    # excluded runs are 1-based, as in the reference
+    evol, err = serving.windowed_means(m, 0.5, excluded=[2])
+    assert np.allclose(evol, [3.0]) and np.allclose(err, [0.0])
+    with pytest.raises(ValueError):
+        serving.windowed_means(np.array([[1.0, 2.0, np.nan]]), 0.0)
