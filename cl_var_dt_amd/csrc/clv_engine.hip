@@ -32,9 +32,7 @@ using namespace clvi;
 
 // Waves per SIMD the fused kernel is compiled for (register budget 512/n),
 // FP32 fields; the FP64 kernel holds twice the data and is built for 2.
-#ifndef CLV_FUSED_WAVES_PER_EU
-#define CLV_FUSED_WAVES_PER_EU 3
-#endif
+constexpr int kFusedWavesPerEU = 3;
 
 // ============================================================== kernels ====
 
@@ -248,7 +246,7 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? 5 : 2) void k_update(BucketAr
 // step; results are bit-identical to the two-pass schedule for every fbatch
 // and line-set count (the same per-line transforms and drift0_site split).
 template <typename T, int X>
-__global__ __launch_bounds__(256, sizeof(T) == 4 ? CLV_FUSED_WAVES_PER_EU : 2) void k_fused(BucketArgs<T> A) {
+__global__ __launch_bounds__(256, sizeof(T) == 4 ? kFusedWavesPerEU : 2) void k_fused(BucketArgs<T> A) {
   extern __shared__ __align__(16) char smem_raw[];
   const int chain = A.chain0 + blockIdx.y;
   const int tile = blockIdx.x;
