@@ -13,6 +13,15 @@
  * stream) and cuFFT replaced by a project Stockham FFT of the same rank and
  * sign conventions.  Fields are complex128 in the reference layout
  * ((((c*Nt+tau)*X+x)*Y+y)*Z+z).  Parameters use the engine's clv_params.
+ *
+ * Pinning (tests/test_oracle.py, tests/test_serving.py): Philox against the
+ * published known-answer vectors; the FFT against numpy.fft and an O(n^2)
+ * DFT; the kinetic term by plane-wave eigenvalues; the drift against an
+ * independent numpy action (-dS/dpsibar, -dS/dpsi) and closed-form special
+ * cases; the dt controller by a closed-form table; max/mean by brute force;
+ * observables by Parseval, the numpy energy density and Pauli-matrix spin
+ * densities on random fields.  The reference ships no golden vectors; no
+ * function here is left unpinned.
  */
 #ifndef CLV_ORACLE_H_
 #define CLV_ORACLE_H_
