@@ -14,8 +14,8 @@
  * sign conventions.  Fields are complex128 in the reference layout
  * ((((c*Nt+tau)*X+x)*Y+y)*Z+z).  Parameters use the engine's clv_params.
  *
- * Pinning (tests/test_oracle.py, tests/test_serving.py): Philox against the
- * published known-answer vectors; the FFT against numpy.fft and an O(n^2)
+ * Pinning (tests/test_oracle.py): Philox against the published
+ * known-answer vectors; the FFT against numpy.fft and an O(n^2)
  * DFT; the kinetic term by plane-wave eigenvalues; the drift against an
  * independent numpy action (-dS/dpsibar, -dS/dpsi) and closed-form special
  * cases; the dt controller by a closed-form table; max/mean by brute force;
