@@ -46,6 +46,25 @@ def measured_traffic(kernel: str, schedule: int, lattice: str, chains: int):
     return None, None
 
 
+# MI355X vector issue: 1024 SIMDs x 16 lanes per cycle (a wave64 instruction
+# takes 4 cycles; packed FP32 off) x 2.4 GHz peak engine clock.
+VALU_PEAK_LANE_INSTR = 1024 * 16 * 2.4e9
+
+
+def measured_valu(kernel: str, x: int, lattice: str, chains: int):
+    """VALU instructions per site-update of `kernel` (SQ_INSTS_VALU per wave
+    lane and tau slice) from the committed counter summary of this workload,
+    or None when no summary matches."""
+    path = os.path.join(ROOT, "profiles", f"valu_{kernel}_{lattice}_fp32.md")
+    if chains != 128 or not os.path.exists(path):
+        return None, None
+    for line in open(path):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) >= 4 and cells[0] == f"`k_{kernel}<float, {x}>`" and cells[3] not in ("", "-"):
+            return float(cells[3]), os.path.relpath(path, ROOT)
+    return None, None
+
+
 def run_batched(args, log):
     import torch
     from cl_var_dt_amd import CLV_FP32, Shape
@@ -346,6 +365,15 @@ def main() -> int:
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dom,
             "avg_launch_ms": round(ms / n, 4), "alg_bytes_per_launch": alg_bytes,
             "traffic_source": tsrc, "schedule": b["schedule"]}
+    # the issue-rate bound the fused step actually meets: its VALU instructions
+    # per site-update (counter summary) at the live launch time
+    ipsu, vsrc = measured_valu(dom, args.lx, f"{args.lx}x{args.lx}x1x{args.nt}", args.chains)
+    valu = None
+    if ipsu:
+        ach = ipsu * b["sites"] / avg_s
+        valu = {"instr_per_site_update": ipsu, "achieved": round(ach / 1e12, 2),
+                "peak": round(VALU_PEAK_LANE_INSTR / 1e12, 2), "unit": "T lane-instr/s",
+                "frac": round(ach / VALU_PEAK_LANE_INSTR, 4), "kernel": dom, "source": vsrc}
     unev = None if args.no_uneven else run_uneven(args, log)
     serv = None if args.no_serving else run_serving(args, log)
     cpu = None if args.no_cpu_baseline else run_cpu_baseline(args, log)
@@ -376,6 +404,7 @@ def main() -> int:
         "batched_over_plain": round(b["su"] / p["su"], 2),
         "kernel_ms": {k: round(v[0], 3) for k, v in b["kt"].items() if v[1]},
         "roofline": roof,
+        "valu_issue": valu,
         "autotune": b["tuning"],
         "uneven_batch": unev,
         "serving": serv,
