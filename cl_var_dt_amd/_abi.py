@@ -16,6 +16,7 @@ CLV_ERR_NOMEM = -3
 CLV_ERR_BUDGET = -4
 CLV_ERR_STATE = -5
 CLV_ERR_NOTFOUND = -6
+CLV_ERR_CANCELLED = -7
 
 CLV_FP64 = 0
 CLV_FP32 = 1
@@ -93,7 +94,8 @@ class ServerInfo(C.Structure):
     _fields_ = [("submitted", C.c_int64), ("completed", C.c_int64), ("running", C.c_int64),
                 ("queued", C.c_int64), ("failed", C.c_int64), ("bytes_in_use", C.c_int64),
                 ("budget_bytes", C.c_int64), ("pools", C.c_int64), ("site_updates", C.c_double),
-                ("busy_seconds", C.c_double)]
+                ("busy_seconds", C.c_double), ("cancelled", C.c_int64),
+                ("pool_setup_seconds", C.c_double)]
 
 
 CLV_RUN_QUEUED = 0
@@ -165,6 +167,7 @@ def _declare(lib):
         "clv_server_poll": (C.c_int, [vp, i64, P(Result)]),
         "clv_server_wait": (C.c_int, [vp, i64, dbl, P(Result)]),
         "clv_server_release": (C.c_int, [vp, i64]),
+        "clv_server_cancel": (C.c_int, [vp, i64]),
         "clv_server_info_get": (C.c_int, [vp, P(ServerInfo)]),
         "clv_server_clock": (dbl, []),
         "clv_server_error": (C.c_char_p, [vp]),
@@ -187,7 +190,7 @@ def exported_symbols() -> list[str]:
         "clv_reset_observables", "clv_autotune", "clv_set_schedule", "clv_get_schedule",
         "clv_set_fused_options", "clv_set_profiling", "clv_kernel_time", "clv_set_graphs",
         "clv_server_create", "clv_server_destroy", "clv_server_submit", "clv_server_poll",
-        "clv_server_wait", "clv_server_release", "clv_server_info_get", "clv_server_clock", "clv_server_error",
+        "clv_server_wait", "clv_server_release", "clv_server_cancel", "clv_server_info_get", "clv_server_clock", "clv_server_error",
     ]
 
 

@@ -1034,8 +1034,12 @@ int64_t chain_bytes(const clv_shape &s, int precision) {
   const int64_t esz = precision == CLV_FP32 ? 8 : 16;
   const int64_t S = int64_t(s.nx) * s.ny * s.nz * s.nt;
   const int64_t Vs = int64_t(s.nx) * s.ny * s.nz;
-  // 2 ping-pong copies of psi and psibar + F accumulators and staging
-  return 4 * S * 4 * esz + 3 * Vs * 8 * (1 + s.nt) + 4096;
+  const int64_t R = int64_t(s.ny) * s.nz;
+  // An upper bound of what setup_bucket allocates per chain: 2 ping-pong
+  // copies of psi and psibar, F accumulators and their per-chunk staging
+  // (at most Nt chunks), the two slots of (max, sum) partials and the
+  // observable partials (at most R tiles per tau), parameters and state.
+  return 4 * S * 4 * esz + 3 * Vs * 8 * (1 + s.nt) + 72 * int64_t(s.nt) * R + 4096;
 }
 
 ChainParams to_device_params(const clv_params &p, int eval_every, int active, int64_t eval_from) {
@@ -1054,7 +1058,7 @@ ChainParams to_device_params(const clv_params &p, int eval_every, int active, in
 // twiddles in the order RowFFT reads them ([k1*N2 + n2] = exp(-2 pi i n2 k1 / L),
 // [L + m2*N1 + k1] = exp(+2 pi i m2 k1 / L)) in the field precision, and
 // -p(k)^2 / L with p of compute_kinetic_part (Propagator_without_cutoff.cu:169-176).
-int upload_row_tables(clv_ctx *ctx, int L, void *d_tw, double *d_p2) {
+void row_tables_host(int L, int precision, std::vector<char> *tw_bytes, std::vector<double> *p2) {
   const int n2s = split_n2(L), n1s = L / n2s;
   std::vector<double> tw(4 * L);
   for (int k1 = 0; k1 < n1s; ++k1)
@@ -1065,19 +1069,28 @@ int upload_row_tables(clv_ctx *ctx, int L, void *d_tw, double *d_p2) {
       tw[2 * (L + n2 * n1s + k1)] = std::cos(a);
       tw[2 * (L + n2 * n1s + k1) + 1] = std::sin(a);
     }
-  std::vector<double> p2(L);
+  p2->assign(L, 0.0);
   for (int k = 0; k < L; ++k) {
     const double p = (k < L / 2 || L == 1) ? 6.283185307179586 * double(k) / double(L)
                                            : -6.283185307179586 * double(L - k) / double(L);
-    p2[k] = -(p * p) / double(L);
+    (*p2)[k] = -(p * p) / double(L);
   }
-  HIPCHK(ctx, hipMemcpy(d_p2, p2.data(), sizeof(double) * L, hipMemcpyHostToDevice));
-  if (ctx->precision == CLV_FP32) {
+  if (precision == CLV_FP32) {
     std::vector<float> twf(tw.begin(), tw.end());
-    HIPCHK(ctx, hipMemcpy(d_tw, twf.data(), sizeof(float) * twf.size(), hipMemcpyHostToDevice));
+    tw_bytes->assign(reinterpret_cast<const char *>(twf.data()),
+                     reinterpret_cast<const char *>(twf.data() + twf.size()));
   } else {
-    HIPCHK(ctx, hipMemcpy(d_tw, tw.data(), sizeof(double) * tw.size(), hipMemcpyHostToDevice));
+    tw_bytes->assign(reinterpret_cast<const char *>(tw.data()),
+                     reinterpret_cast<const char *>(tw.data() + tw.size()));
   }
+}
+
+int upload_row_tables(clv_ctx *ctx, int L, void *d_tw, double *d_p2) {
+  std::vector<char> tw;
+  std::vector<double> p2;
+  row_tables_host(L, ctx->precision, &tw, &p2);
+  HIPCHK(ctx, hipMemcpy(d_p2, p2.data(), sizeof(double) * L, hipMemcpyHostToDevice));
+  HIPCHK(ctx, hipMemcpy(d_tw, tw.data(), tw.size(), hipMemcpyHostToDevice));
   return CLV_OK;
 }
 
@@ -1109,14 +1122,17 @@ int enable_kt(clv_ctx *ctx, Bucket &b) {
   return CLV_OK;
 }
 
-int setup_bucket(clv_ctx *ctx, Bucket &b) {
+// Tiling of a bucket of b.shape (rows and tiles per tau-slice, threads per
+// workgroup, observable tau-chunks, drift-statistics tau slices, fused kernel
+// options and schedule); no device work.
+void plan_bucket(Bucket &b, int precision) {
   const clv_shape &sh = b.shape;
   b.X = sh.nx;
   b.R = sh.ny * sh.nz;
   b.Nt = sh.nt;
   b.Vs = int64_t(sh.nx) * sh.ny * sh.nz;
   b.S = b.Vs * sh.nt;
-  b.esz = (ctx->precision == CLV_FP32) ? 8 : 16;
+  b.esz = (precision == CLV_FP32) ? 8 : 16;
   b.rows = pick_rows(b.X, b.R, std::max(kTileSites, b.X));
   // the update kernel stages 8 lines per row: keep its LDS within 64 KB
   const auto update_lds = [&](int rows) {
@@ -1137,6 +1153,24 @@ int setup_bucket(clv_ctx *ctx, Bucket &b) {
   b.tcs = 1;  // tau slices per drift-stats workgroup: fill the transform batch
   for (int q = 1; q <= 4 && q * b.rows * b.X <= 1024; ++q)
     if (b.Nt % q == 0) b.tcs = q;
+  b.part_slot = size_t(2) * b.nch * b.Nt * b.ntiles;
+  b.fused_sets = b.fused_sets_max = (fused_lds_bytes(b, 2, 1) <= 65536) ? 2 : 1;
+  b.fused_batch = 1;
+  // Untuned default: one line set with K_0 lines in batches of 4 (else 2)
+  // taus.  Measured on MI355X it is the fastest option or within 3% of it
+  // from 16^3 to 256^2 rows (up to 16% ahead of unbatched).
+  for (int B = 4; B >= 2; B /= 2)
+    if (b.fused_batch == 1 && B <= b.Nt && fused_lds_bytes(b, 1, B) <= 65536) {
+      b.fused_sets = 1;
+      b.fused_batch = B;
+    }
+  b.fused_lds = fused_lds_bytes(b, b.fused_sets, b.fused_batch);
+  if (b.fused_lds > 65536) b.fused_lds = 0;
+  b.schedule = auto_schedule(b, b.nch);
+}
+
+int setup_bucket(clv_ctx *ctx, Bucket &b) {
+  plan_bucket(b, ctx->precision);
   const size_t field = size_t(b.nch) * size_t(b.S) * 4 * b.esz;
   int rc;
   for (int k = 0; k < 2; ++k) {
@@ -1147,7 +1181,6 @@ int setup_bucket(clv_ctx *ctx, Bucket &b) {
   if ((rc = dalloc(ctx, b, reinterpret_cast<void **>(&b.d_st), 2 * sizeof(ChainState) * b.nch))) return rc;
   if ((rc = dalloc(ctx, b, &b.d_tw, 2 * b.esz * b.X))) return rc;
   if ((rc = dalloc(ctx, b, reinterpret_cast<void **>(&b.d_p2), sizeof(double) * b.X))) return rc;
-  b.part_slot = size_t(2) * b.nch * b.Nt * b.ntiles;
   if ((rc = dalloc(ctx, b, reinterpret_cast<void **>(&b.d_part), sizeof(double) * 2 * b.part_slot)))
     return rc;
   if ((rc = dalloc(ctx, b, reinterpret_cast<void **>(&b.d_stage),
@@ -1173,19 +1206,6 @@ int setup_bucket(clv_ctx *ctx, Bucket &b) {
   b.host_eval.assign(b.nch, 0);
   b.host_eval_from.assign(b.nch, 0);
   b.host_active.assign(b.nch, 1);
-  b.fused_sets = b.fused_sets_max = (fused_lds_bytes(b, 2, 1) <= 65536) ? 2 : 1;
-  b.fused_batch = 1;
-  // Untuned default: one line set with K_0 lines in batches of 4 (else 2)
-  // taus.  Measured on MI355X it is the fastest option or within 3% of it
-  // from 16^3 to 256^2 rows (up to 16% ahead of unbatched).
-  for (int B = 4; B >= 2; B /= 2)
-    if (b.fused_batch == 1 && B <= b.Nt && fused_lds_bytes(b, 1, B) <= 65536) {
-      b.fused_sets = 1;
-      b.fused_batch = B;
-    }
-  b.fused_lds = fused_lds_bytes(b, b.fused_sets, b.fused_batch);
-  if (b.fused_lds > 65536) b.fused_lds = 0;
-  b.schedule = auto_schedule(b, b.nch);
   return CLV_OK;
 }
 
@@ -1305,6 +1325,19 @@ int chain_set_active(clv_ctx *ctx, Bucket &b, int local, const clv_params &p, in
   HIPCHK(ctx, hipMemcpyAsync(b.d_prm + local, &d, sizeof(d), hipMemcpyHostToDevice, ctx->stream));
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   b.host_active[local] = active ? 1 : 0;
+  return CLV_OK;
+}
+
+int chains_deactivate(clv_ctx *ctx, Bucket &b, const std::vector<int> &locals) {
+  if (locals.empty()) return CLV_OK;
+  // zeroed records: active = 0, which is all the kernels read of a free slot
+  std::vector<ChainParams> z(locals.size());
+  std::memset(z.data(), 0, sizeof(ChainParams) * z.size());
+  for (size_t i = 0; i < locals.size(); ++i)
+    HIPCHK(ctx, hipMemcpyAsync(b.d_prm + locals[i], &z[i], sizeof(ChainParams), hipMemcpyHostToDevice,
+                               ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));  // z must outlive the copies
+  for (int l : locals) b.host_active[l] = 0;
   return CLV_OK;
 }
 
@@ -1503,6 +1536,9 @@ int clv_set_params(clv_ctx *ctx, int32_t chain, const clv_params *p) {
                                    b.host_eval_from[local]);
   HIPCHK(ctx, hipMemcpyAsync(b.d_prm + local, &d, sizeof(d), hipMemcpyHostToDevice, ctx->stream));
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  // The fused kernel left next step's |D_0| partials computed with the old
+  // parameters (coefficients and kinetic table scale): recompute them.
+  b.primed = false;
   return CLV_OK;
 }
 

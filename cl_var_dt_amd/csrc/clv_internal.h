@@ -140,6 +140,9 @@ int chain_admit(clv_ctx *ctx, Bucket &b, int local, const clv_params &p, int eva
 void chain_evaluate(clv_ctx *ctx, Bucket &b, int local);
 int chain_set_active(clv_ctx *ctx, Bucket &b, int local, const clv_params &p, int eval_every,
                      bool active);
+// Free the slots `locals` (their kernels exit at once): one batch of copies
+// and one synchronisation.
+int chains_deactivate(clv_ctx *ctx, Bucket &b, const std::vector<int> &locals);
 int chain_read_observables(clv_ctx *ctx, Bucket &b, int local, clv_observables *out, double *Fx,
                            double *Fy, double *Fz);
 void stats_from_state(const ChainState &st, clv_step_stats *out);

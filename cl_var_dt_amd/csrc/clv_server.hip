@@ -1,21 +1,28 @@
 // clv_server.hip -- the serving loop: continuous batching of Langevin runs.
 //
-// A worker thread owns the device.  Runs wait in a FIFO queue; the worker
-// admits them into free slots of an idle pool of their shape, creating a pool
-// when the HBM budget allows (and releasing empty pools of other shapes when
-// it does not).  Each pool is a bucket of equal-shape chain slots with its own
-// HIP stream, so pools of different shapes run concurrently on the GPU.  Pools
-// tick independently: an idle pool with active slots is given k steps (the
-// smallest remaining step count of its runs, capped so that a tick lasts about
-// kTickTargetMs), the worker polls the tick's completion event, and a pool
-// whose tick ended is accounted, its finished runs read back and freed, and
-// it is refilled and relaunched without waiting for the other pools.  Every
-// run stops exactly after its own nsteps.  Nothing a run computes depends on its slot or on
-// its neighbours: noise is keyed by the run's seed and step, reductions are
+// A worker thread owns the device.  Clients submit runs into an inbox; the
+// worker moves them to its own waiting list and admits them into free slots of
+// an idle pool of their shape, creating a pool when the HBM budget allows (and
+// releasing empty pools of other shapes when it does not).  Each pool is a
+// bucket of equal-shape chain slots with its own HIP stream, so pools of
+// different shapes run concurrently on the GPU.  Pools tick independently: an
+// idle pool with active slots is given k steps (the smallest remaining step
+// count of its runs, capped so that a tick lasts about kTickTargetMs), the
+// worker polls the tick's completion event, and a pool whose tick ended is
+// accounted, its finished and cancelled runs read back and freed, and it is
+// refilled and relaunched without waiting for the other pools.  Every run stops
+// exactly after its own nsteps.  Nothing a run computes depends on its slot or
+// on its neighbours: noise is keyed by the run's seed and step, reductions are
 // per chain and in a fixed order.
+//
+// Locking: the client mutex guards the inbox, the run records' results and the
+// counters, and is never held across device work (pool creation, admission,
+// tick launches, read-back): clients that submit, poll or wait never queue
+// behind an allocation or a stream synchronisation.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
@@ -42,9 +49,12 @@ double now_s() {
 struct Run {
   int64_t id = 0;
   clv_request req{};
-  clv_result res{};
+  clv_result res{};             // guarded by the server mutex
+  std::atomic<bool> cancel{false};
+  // worker only
   int pool = -1, slot = -1;
   int64_t remaining = 0;
+  int64_t failed_gen = -1;      // generation at which creating its pool last failed
 };
 
 struct Pool {
@@ -76,59 +86,78 @@ struct clv_server {
   int64_t budget = 0;
   mutable std::mutex mu;
   std::condition_variable cv_work, cv_done;
-  std::deque<Run *> queue;
+  std::deque<Run *> inbox;                       // guarded by mu
   std::map<int64_t, std::unique_ptr<Run>> runs;  // guarded by mu
-  std::vector<std::unique_ptr<Pool>> pools;
   int64_t next_id = 1;
   bool stop = false;
-  std::thread worker;
-  std::string err;
-  clv_server_info info{};
+  std::string err;                               // guarded by mu
+  clv_server_info info{};                        // guarded by mu
+  // worker only
+  std::deque<Run *> waiting;
+  std::vector<std::unique_ptr<Pool>> pools;
   int64_t bytes_in_use = 0;
+  int64_t gen = 0;              // bumps whenever a slot or pool memory is freed
+  double setup_s = 0.0;         // time spent creating pools
+  std::thread worker;
 };
 
 namespace {
 
 int64_t pool_bytes(const Pool &p) { return p.ctx.bytes; }
 
+void set_error(clv_server *s, const std::string &msg) {
+  std::lock_guard<std::mutex> g(s->mu);
+  s->err = msg;
+}
+
 // Create a pool for `shape` with as many slots as the target size and the
-// remaining budget allow.  Called by the worker only.
-Pool *make_pool(clv_server *s, const clv_shape &shape) {
+// remaining budget allow.  nullptr with *err = 0 when the budget has no room
+// (wait for runs to finish), with *err < 0 when the device refused.
+Pool *make_pool(clv_server *s, const clv_shape &shape, int *err) {
+  *err = 0;
   const int64_t per = chain_bytes(shape, s->precision);
-  int64_t room = s->budget - s->bytes_in_use;
+  const int64_t room = s->budget - s->bytes_in_use;
   int cap = int(std::min<int64_t>(kPoolMaxSlots, std::max<int64_t>(1, kPoolTargetBytes / per)));
   cap = int(std::min<int64_t>(cap, room / per));
   if (cap < 1) return nullptr;
+  const double t0 = now_s();
   std::unique_ptr<Pool> p(new Pool());
   p->ctx.device = s->device;
   p->ctx.precision = s->precision;
-  if (hipStreamCreateWithFlags(&p->ctx.stream, hipStreamNonBlocking) != hipSuccess) return nullptr;
+  if (hipStreamCreateWithFlags(&p->ctx.stream, hipStreamNonBlocking) != hipSuccess) {
+    (void)hipGetLastError();
+    *err = CLV_ERR_HIP;
+    set_error(s, "clv_server: hipStreamCreate failed");
+    return nullptr;
+  }
   p->ctx.buckets.emplace_back();
   Bucket &b = p->ctx.buckets.back();
   b.shape = shape;
   b.nch = cap;
-  if (setup_bucket(&p->ctx, b) != CLV_OK) {
+  const int rc = setup_bucket(&p->ctx, b);
+  if (rc != CLV_OK) {
     free_bucket(&p->ctx, b);
     (void)hipStreamDestroy(p->ctx.stream);
-    s->err = p->ctx.err;
+    *err = rc;
+    set_error(s, "clv_server: creating a pool of " + std::to_string(cap) + " slots: " + p->ctx.err);
     return nullptr;
   }
+  // setup_bucket zeroes the parameter records, so every slot starts inactive
   b.schedule = 1;
   b.host_active.assign(b.nch, 0);
-  // slots start inactive: mark every chain inactive on the device
-  clv_params dflt;
-  clv_default_params(&dflt);
-  for (int i = 0; i < b.nch; ++i) (void)chain_set_active(&p->ctx, b, i, dflt, 0, false);
   p->slot_run.assign(cap, nullptr);
   if (hipEventCreate(&p->ev_start) != hipSuccess || hipEventCreate(&p->ev_done) != hipSuccess) {
+    (void)hipGetLastError();
     free_bucket(&p->ctx, b);
     (void)hipStreamDestroy(p->ctx.stream);
-    s->err = "clv_server: hipEventCreate failed";
+    *err = CLV_ERR_HIP;
+    set_error(s, "clv_server: hipEventCreate failed");
     return nullptr;
   }
   p->alive = true;
   s->bytes_in_use += pool_bytes(*p);
   s->pools.push_back(std::move(p));
+  s->setup_s += now_s() - t0;
   return s->pools.back().get();
 }
 
@@ -144,29 +173,44 @@ void free_pool(clv_server *s, Pool &p) {
   p.ctx.buckets.clear();
   p.alive = false;
   p.busy = false;
+  s->gen += 1;
 }
 
-// Find a free slot for `shape` (idle pool of that shape, new pool, or after
-// releasing empty pools of other shapes).  Returns false if it has to wait;
-// a pool with a tick in flight takes new runs when the tick has ended.
-bool place(clv_server *s, const clv_shape &shape, int *pool, int *slot) {
+bool any_pool_running(const clv_server *s) {
+  for (const auto &p : s->pools)
+    if (p->alive && (p->busy || p->nactive > 0)) return true;
+  return false;
+}
+
+constexpr int kPlaced = 1, kWait = 0;  // place(): negative = fail the run with that code
+
+// Find a free slot for run r (idle pool of its shape, new pool, or after
+// releasing empty pools of other shapes).  kWait if it has to wait: a pool
+// with a tick in flight takes new runs when the tick has ended, and runs in
+// flight free budget when they finish.  A device allocation failure fails the
+// run when nothing in flight could free memory; otherwise the run waits and
+// the allocation is retried only after some memory has been freed.
+int place(clv_server *s, Run &r, int *pool, int *slot) {
+  const clv_shape &shape = r.req.shape;
   bool shape_busy = false;
   for (size_t k = 0; k < s->pools.size(); ++k) {
     Pool &p = *s->pools[k];
     if (!p.alive || !same_shape(p.ctx.buckets[0].shape, shape)) continue;
     if (p.busy) {
-      for (Run *r : p.slot_run) shape_busy |= (r == nullptr);
+      for (Run *q : p.slot_run) shape_busy |= (q == nullptr);
       continue;
     }
     for (size_t i = 0; i < p.slot_run.size(); ++i)
       if (p.slot_run[i] == nullptr) {
         *pool = int(k);
         *slot = int(i);
-        return true;
+        return kPlaced;
       }
   }
-  if (shape_busy) return false;  // a slot frees up at the end of that pool's tick
-  Pool *np = make_pool(s, shape);
+  if (shape_busy) return kWait;        // a slot frees up at the end of that pool's tick
+  if (r.failed_gen == s->gen) return kWait;  // nothing freed since the last refusal
+  int err = 0;
+  Pool *np = make_pool(s, shape, &err);
   if (!np) {
     // release idle pools of other shapes, then try once more
     bool freed = false;
@@ -175,46 +219,94 @@ bool place(clv_server *s, const clv_shape &shape, int *pool, int *slot) {
         free_pool(s, *pp);
         freed = true;
       }
-    if (freed) np = make_pool(s, shape);
+    if (freed) np = make_pool(s, shape, &err);
   }
-  if (!np) return false;
+  if (!np) {
+    if (err == 0) return kWait;                 // budget: runs in flight will free it
+    if (!any_pool_running(s)) return err;       // nothing will ever free device memory
+    r.failed_gen = s->gen;
+    return kWait;
+  }
   for (size_t k = 0; k < s->pools.size(); ++k)
     if (s->pools[k].get() == np) {
       *pool = int(k);
       *slot = 0;
-      return true;
+      return kPlaced;
     }
-  return false;
+  return kWait;
 }
 
-void finish_run(clv_server *s, Run &r) {
-  Pool &p = *s->pools[r.pool];
-  Bucket &b = p.ctx.buckets[0];
-  clv_observables obs;
-  ChainState st;
-  int rc = chain_read_observables(&p.ctx, b, r.slot, &obs, nullptr, nullptr, nullptr);
-  if (rc == CLV_OK && obs.count == 0) {
-    // a run that reached no evaluation step is evaluated once, on its final fields
-    chain_evaluate(&p.ctx, b, r.slot);
-    rc = chain_read_observables(&p.ctx, b, r.slot, &obs, nullptr, nullptr, nullptr);
-  }
-  if (rc == CLV_OK) rc = read_state(&p.ctx, b, r.slot, &st);
-  if (rc == CLV_OK) rc = chain_set_active(&p.ctx, b, r.slot, r.req.params, 0, false);
+// A run that never reached a slot ends with `status` (worker; takes the lock).
+void conclude_unplaced(clv_server *s, Run &r, int status, const std::string &msg) {
   std::lock_guard<std::mutex> g(s->mu);
-  if (rc == CLV_OK) {
-    r.res.obs = obs;
-    stats_from_state(st, &r.res.stats);
-    r.res.status = CLV_RUN_DONE;
-    s->info.completed += 1;
-  } else {
-    r.res.status = rc;
-    s->info.failed += 1;
-    s->err = p.ctx.err;
-  }
+  r.res.status = status;
   r.res.t_done = now_s();
-  p.slot_run[r.slot] = nullptr;
-  p.nactive -= 1;
-  s->info.running -= 1;
+  if (!msg.empty()) s->err = msg;
+  s->info.queued -= 1;
+  if (status == CLV_ERR_CANCELLED) s->info.cancelled += 1;
+  else s->info.failed += 1;
+  s->cv_done.notify_all();
+}
+
+// Read back and free the slots of pool p (idle) whose runs have finished their
+// steps or were cancelled: observables (a finished run that reached no
+// evaluation step is evaluated once, on its final fields) and step
+// statistics, then one batched deactivation.  Device work without the lock;
+// the results are published under it.
+void reap(clv_server *s, Pool &p) {
+  if (!p.alive || p.busy) return;
+  std::vector<Run *> out;
+  for (Run *r : p.slot_run)
+    if (r && (r->remaining <= 0 || r->cancel.load())) out.push_back(r);
+  if (out.empty()) return;
+  // a run that completed its steps by this boundary ends as done, cancelled or not
+  std::vector<char> cancelled(out.size());
+  for (size_t i = 0; i < out.size(); ++i) cancelled[i] = out[i]->remaining > 0;
+  Bucket &b = p.ctx.buckets[0];
+  std::vector<clv_observables> obs(out.size());
+  std::vector<ChainState> st(out.size());
+  std::vector<int> rcs(out.size(), CLV_OK), locals;
+  for (size_t i = 0; i < out.size(); ++i) {
+    Run &r = *out[i];
+    int rc = CLV_OK;
+    std::memset(&obs[i], 0, sizeof(obs[i]));
+    if (!cancelled[i]) {
+      rc = chain_read_observables(&p.ctx, b, r.slot, &obs[i], nullptr, nullptr, nullptr);
+      if (rc == CLV_OK && obs[i].count == 0) {
+        chain_evaluate(&p.ctx, b, r.slot);
+        rc = chain_read_observables(&p.ctx, b, r.slot, &obs[i], nullptr, nullptr, nullptr);
+      }
+    }
+    if (rc == CLV_OK) rc = read_state(&p.ctx, b, r.slot, &st[i]);
+    rcs[i] = rc;
+    locals.push_back(r.slot);
+  }
+  const int rcd = chains_deactivate(&p.ctx, b, locals);
+  std::lock_guard<std::mutex> g(s->mu);
+  for (size_t i = 0; i < out.size(); ++i) {
+    Run &r = *out[i];
+    const int rc = rcs[i] != CLV_OK ? rcs[i] : rcd;
+    if (rc == CLV_OK) {
+      r.res.obs = obs[i];
+      stats_from_state(st[i], &r.res.stats);
+    }
+    if (rc != CLV_OK) {
+      r.res.status = rc;
+      s->info.failed += 1;
+      s->err = p.ctx.err;
+    } else if (cancelled[i]) {
+      r.res.status = CLV_ERR_CANCELLED;
+      s->info.cancelled += 1;
+    } else {
+      r.res.status = CLV_RUN_DONE;
+      s->info.completed += 1;
+    }
+    r.res.t_done = now_s();
+    p.slot_run[r.slot] = nullptr;
+    p.nactive -= 1;
+    s->info.running -= 1;
+  }
+  s->gen += 1;
   s->cv_done.notify_all();
 }
 
@@ -222,52 +314,72 @@ void worker_main(clv_server *s) {
   (void)hipSetDevice(s->device);
   double busy_since = -1.0;
   for (;;) {
-    // ---- admission (tick boundary) ----
-    std::vector<Run *> admitted;
+    // ---- new submissions (the only part under the lock) ----
     {
       std::unique_lock<std::mutex> lk(s->mu);
-      bool any_active = false;
-      for (auto &p : s->pools) any_active |= (p->alive && p->nactive > 0);
-      if (!any_active && s->queue.empty()) {
+      const bool any_active = any_pool_running(s);
+      if (!any_active && s->waiting.empty() && s->inbox.empty()) {
         if (busy_since >= 0) {
           s->info.busy_seconds += now_s() - busy_since;
           busy_since = -1.0;
         }
-        s->cv_work.wait(lk, [&] { return s->stop || !s->queue.empty(); });
+        s->cv_work.wait(lk, [&] { return s->stop || !s->inbox.empty(); });
       }
       if (s->stop) break;
-      for (auto it = s->queue.begin(); it != s->queue.end();) {
-        Run &r = **it;
-        const int64_t per = chain_bytes(r.req.shape, s->precision);
-        if (per > s->budget) {
-          r.res.status = CLV_ERR_BUDGET;
-          r.res.t_done = now_s();
-          s->info.failed += 1;
-          s->info.queued -= 1;
-          it = s->queue.erase(it);
-          s->cv_done.notify_all();
-          continue;
+      while (!s->inbox.empty()) {
+        s->waiting.push_back(s->inbox.front());
+        s->inbox.pop_front();
+      }
+    }
+    // ---- finished and cancelled runs of idle pools free their slots ----
+    for (auto &pp : s->pools) reap(s, *pp);
+    // ---- admission (tick boundary) ----
+    std::vector<Run *> admitted;
+    for (auto it = s->waiting.begin(); it != s->waiting.end();) {
+      Run &r = **it;
+      if (r.cancel.load()) {
+        conclude_unplaced(s, r, CLV_ERR_CANCELLED, "");
+        it = s->waiting.erase(it);
+        continue;
+      }
+      const int64_t per = chain_bytes(r.req.shape, s->precision);
+      if (per > s->budget) {
+        conclude_unplaced(s, r, CLV_ERR_BUDGET,
+                          "clv_server: run " + std::to_string(r.id) + " needs " + std::to_string(per) +
+                              " bytes, more than the HBM budget");
+        it = s->waiting.erase(it);
+        continue;
+      }
+      int pool = -1, slot = -1;
+      const int pr = place(s, r, &pool, &slot);
+      if (pr == kWait) {
+        ++it;
+        continue;
+      }
+      if (pr < 0) {
+        std::string msg;
+        {
+          std::lock_guard<std::mutex> g(s->mu);
+          msg = s->err;
         }
-        int pool = -1, slot = -1;
-        if (!place(s, r.req.shape, &pool, &slot)) {
-          ++it;
-          continue;
-        }
-        r.pool = pool;
-        r.slot = slot;
-        r.remaining = r.req.nsteps;
-        s->pools[pool]->slot_run[slot] = &r;
-        s->pools[pool]->nactive += 1;
+        conclude_unplaced(s, r, pr, msg);
+        it = s->waiting.erase(it);
+        continue;
+      }
+      r.pool = pool;
+      r.slot = slot;
+      r.remaining = r.req.nsteps;
+      s->pools[pool]->slot_run[slot] = &r;
+      s->pools[pool]->nactive += 1;
+      {
+        std::lock_guard<std::mutex> g(s->mu);
         r.res.status = CLV_RUN_RUNNING;
         r.res.t_admit = now_s();
         s->info.queued -= 1;
         s->info.running += 1;
-        admitted.push_back(&r);
-        it = s->queue.erase(it);
       }
-      s->info.pools = 0;
-      for (auto &p : s->pools) s->info.pools += p->alive ? 1 : 0;
-      s->info.bytes_in_use = s->bytes_in_use;
+      admitted.push_back(&r);
+      it = s->waiting.erase(it);
     }
     for (Run *rp : admitted) {
       Run &r = *rp;
@@ -284,9 +396,17 @@ void worker_main(clv_server *s) {
         s->info.running -= 1;
         s->info.failed += 1;
         s->cv_done.notify_all();
-      } else if (r.remaining == 0) {
-        finish_run(s, r);
       }
+    }
+    // zero-step runs and runs cancelled during admission end before any tick
+    if (!admitted.empty())
+      for (auto &pp : s->pools) reap(s, *pp);
+    {
+      std::lock_guard<std::mutex> g(s->mu);
+      s->info.pools = 0;
+      for (auto &p : s->pools) s->info.pools += p->alive ? 1 : 0;
+      s->info.bytes_in_use = s->bytes_in_use;
+      s->info.pool_setup_seconds = s->setup_s;
     }
     // ---- launch a tick on every idle pool with active slots ----
     bool launched = false;
@@ -310,7 +430,6 @@ void worker_main(clv_server *s) {
     // ---- account the pools whose tick has ended ----
     bool progressed = false;
     double su = 0.0;
-    std::vector<Run *> done;
     for (auto &pp : s->pools) {
       Pool &p = *pp;
       if (!p.alive || !p.busy) continue;
@@ -323,16 +442,13 @@ void worker_main(clv_server *s) {
       progressed = true;
       su += double(p.nactive) * double(p.ctx.buckets[0].S) * double(p.k_inflight);
       for (Run *r : p.slot_run)
-        if (r) {
-          r->remaining -= p.k_inflight;
-          if (r->remaining <= 0) done.push_back(r);
-        }
+        if (r) r->remaining -= p.k_inflight;
+      reap(s, p);
     }
     if (progressed) {
       std::lock_guard<std::mutex> g(s->mu);
       s->info.site_updates += su;
     }
-    for (Run *r : done) finish_run(s, *r);
     if (!progressed && !launched && admitted.empty())
       std::this_thread::sleep_for(std::chrono::microseconds(50));
   }
@@ -395,25 +511,30 @@ int clv_server_destroy(clv_server *s) {
 }
 
 int clv_server_submit(clv_server *s, const clv_request *req, int64_t *id) {
+  const double t_arrival = now_s();  // before any lock: blocked time counts as latency
   if (!s || !req) return CLV_ERR_ARG;
   const clv_shape &sh = req->shape;
+  const clv_params &pr = req->params;
   if (sh.nx <= 0 || sh.ny <= 0 || sh.nz <= 0 || sh.nt < 2 || !supported_x(sh.nx) ||
-      req->nsteps < 0 || req->params.kin_dims != 1 || !(req->params.DT > 0.0) ||
-      !(req->params.eps > 0.0) || !(req->params.a_x > 0.0) || req->n0 < 0 || req->var < 0) {
+      int64_t(sh.nx) * sh.ny * sh.nz * sh.nt >= (int64_t(1) << 31) || req->nsteps < 0 ||
+      pr.kin_dims != 1 || !(pr.DT > 0.0) || !(pr.eps > 0.0) || !(pr.a_x > 0.0) ||
+      (pr.conj_drift_mode != CLV_CONJ_DRIFT_REFERENCE && pr.conj_drift_mode != CLV_CONJ_DRIFT_ACTION) ||
+      !(req->n0 >= 0) || !(req->var >= 0)) {
     std::lock_guard<std::mutex> g(s->mu);
-    s->err = "clv_server_submit: invalid request (shape, nsteps, kin_dims, DT, eps, a_x, n0, var)";
+    s->err = "clv_server_submit: invalid request (shape, nsteps, kin_dims, DT, eps, a_x, "
+             "conj_drift_mode, n0, var)";
     return CLV_ERR_ARG;
   }
-  std::lock_guard<std::mutex> g(s->mu);
   std::unique_ptr<Run> r(new Run());
-  r->id = s->next_id++;
   r->req = *req;
-  r->res.id = r->id;
   r->res.status = CLV_RUN_QUEUED;
-  r->res.t_submit = now_s();
+  r->res.t_submit = t_arrival;
   Run *rp = r.get();
+  std::lock_guard<std::mutex> g(s->mu);
+  r->id = s->next_id++;
+  r->res.id = r->id;
   s->runs[rp->id] = std::move(r);
-  s->queue.push_back(rp);
+  s->inbox.push_back(rp);
   s->info.submitted += 1;
   s->info.queued += 1;
   if (id) *id = rp->id;
@@ -461,11 +582,25 @@ int clv_server_release(clv_server *s, int64_t id) {
   return CLV_OK;
 }
 
+int clv_server_cancel(clv_server *s, int64_t id) {
+  if (!s) return CLV_ERR_ARG;
+  std::lock_guard<std::mutex> g(s->mu);
+  auto it = s->runs.find(id);
+  if (it == s->runs.end()) return CLV_ERR_NOTFOUND;
+  const int st = it->second->res.status;
+  if (st != CLV_RUN_QUEUED && st != CLV_RUN_RUNNING) {
+    s->err = "clv_server_cancel: run " + std::to_string(id) + " has already finished";
+    return CLV_ERR_STATE;
+  }
+  it->second->cancel.store(true);
+  s->cv_work.notify_all();
+  return CLV_OK;
+}
+
 int clv_server_info_get(clv_server *s, clv_server_info *out) {
   if (!s || !out) return CLV_ERR_ARG;
   std::lock_guard<std::mutex> g(s->mu);
   *out = s->info;
-  out->bytes_in_use = s->bytes_in_use;
   out->budget_bytes = s->budget;
   return CLV_OK;
 }

@@ -3,10 +3,12 @@ mu search of the reference's orchestration as a client of it.
 
 `Server` wraps clv_server_* (include/clv.h): runs are submitted at any time,
 admitted into device-resident slot pools by HBM budget while others are in
-flight, and return their observables and time-to-result.  `secant_mu_search`
-restates the secant update of mylib.py:561-584 (new_mu / check_to_stop) with
-follow-up runs issued to the server toward a target density; `windowed_means`
-restates the burn-in-cut running means of mylib.py:384-404.
+flight, and return their observables and time-to-result.  `secant_method`
+restates the search loop of Secant_method_function (mylib.py:599-706) with its
+new_mu / check_to_stop / change_guess (mylib.py:549-593), each density a mean
+over independent server runs with its standard error; `secant_mu_search` is
+the bare secant iteration on the same new_mu; `windowed_means` restates the
+burn-in-cut running means of mylib.py:384-404.
 """
 from __future__ import annotations
 
@@ -72,6 +74,13 @@ class Server:
             raise _abi.ClvError(st, lib().clv_server_error(self._h).decode())
         return rid.value
 
+    def cancel(self, rid: int) -> None:
+        """Cancel a queued or running run (clv_server_cancel); its wait then
+        raises ClvError with status CLV_ERR_CANCELLED."""
+        st = lib().clv_server_cancel(self._h, rid)
+        if st != _abi.CLV_OK:
+            raise _abi.ClvError(st, lib().clv_server_error(self._h).decode())
+
     def poll(self, rid: int) -> Result:
         r = Result()
         _abi.check(lib().clv_server_poll(self._h, rid, C.byref(r)))
@@ -104,19 +113,134 @@ class Server:
         return i
 
 
-def new_mu(r: Sequence[float], mu: Sequence[float]) -> float:
-    """Secant step mu_{n+1} = (r_{n+1} mu_n - r_n mu_{n+1}) / (r_{n+1} - r_n) on
-    the residuals r = observable - target (mylib.py:561-580)."""
-    r0, r1 = r[-2], r[-1]
-    m0, m1 = mu[-2], mu[-1]
-    if r1 == r0:
-        return m1
-    return (r1 * m0 - r0 * m1) / (r1 - r0)
+class JError(ValueError):
+    """No earlier point left for a secant step after the first (mylib.py:549-553)."""
+
+
+class GuessError(ValueError):
+    """The first secant step from the two initial guesses failed (mylib.py:555-558)."""
+
+
+def new_mu(r: Sequence[float], mu: Sequence[float], i: int | None = None) -> float:
+    """Secant step of new_mu (mylib.py:561-580) on residuals r = observable - target:
+        mu_new = (r[i+1] mu[j] - r[j] mu[i+1]) / (r[i+1] - r[j]),  j = i, i-1, ...
+    i defaults to the last pair (i = len(r) - 2).  A step is refused and j moves
+    to an older point when mu_new lies in [10 max(mu), 0] -- the reference's
+    guard, kept as written: it can only fire when no mu so far is positive --
+    or, unlike the reference, when r[i+1] == r[j] (there numpy returns inf or
+    nan and the search goes on with that mu).  With no point left: GuessError on
+    the first step (i = 0), JError after it."""
+    i = len(r) - 2 if i is None else i
+    hi = 10.0 * max(mu)
+    j = i
+    while True:
+        den = r[i + 1] - r[j]
+        new = (r[i + 1] * mu[j] - r[j] * mu[i + 1]) / den if den != 0 else float("nan")
+        if den == 0 or (hi <= new <= 0.0):
+            j -= 1
+            if j < 0:
+                if i == 0:
+                    raise GuessError("Bad guess.")
+                raise JError("j too small. Secant method failed")
+            continue
+        return new
 
 
 def check_to_stop(residual: float, err: float) -> bool:
     """mylib.py:582-584: stop once |observable - target| < err."""
     return abs(residual) < err
+
+
+def change_guess(r: Sequence[float], guess: float) -> float:
+    """change_guess (mylib.py:591-593): both initial residuals positive halves
+    the guess.  Otherwise the reference returns None and its caller fails on
+    unpacking it (and it also unpacks the halved guess into (guess, delta));
+    here delta is kept and the other case raises GuessError."""
+    if r[0] > 0 and r[1] > 0:
+        return 0.5 * guess
+    raise GuessError("Bad guess: the initial residuals do not both lie above the target")
+
+
+def _value_err(v):
+    return (float(v[0]), float(v[1])) if isinstance(v, (tuple, list)) else (float(v), None)
+
+
+def secant_mu_search(run_density: Callable[[float], float], mu0: float, mu1: float,
+                     target: float = 1.0, tol: float = 1e-3, max_iter: int = 15):
+    """Secant iteration of mu with density(mu) = target, each evaluation a
+    server run.  run_density returns the density or (density, standard
+    error); with an error the search stops when the residual is inside it
+    (check_to_stop(rho - 1, rho_err), mylib.py:687), else inside `tol`.
+    Returns (mu, history of (mu, residual))."""
+    mus, res = [mu0, mu1], []
+    errs = []
+    for m in mus:
+        v, e = _value_err(run_density(m))
+        res.append(v - target)
+        errs.append(tol if e is None else e)
+    for _ in range(max_iter):
+        if check_to_stop(res[-1], errs[-1]):
+            break
+        m = new_mu(res, mus)
+        mus.append(m)
+        v, e = _value_err(run_density(m))
+        res.append(v - target)
+        errs.append(tol if e is None else e)
+    return mus[-1], list(zip(mus, res))
+
+
+def secant_method(run_many: Callable[[Sequence[float]], Sequence[tuple]], guess: float,
+                  delta: float, target: float = 1.0, max_i: int = 15):
+    """The search loop of Secant_method_function (mylib.py:599-706).
+
+    run_many(mus) returns one (density, standard error) per mu; the first two
+    points mu = round((1 +- delta) guess, 5) are run together, as the reference
+    runs them in two threads.  A GuessError on the first step restarts from
+    change_guess; a JError later ends the search; the search stops when the
+    last density is within its error of the target (check_to_stop) or at
+    i == 15 as in the reference.  Returns (mu, history of (mu, density, err),
+    converged)."""
+    mu_s = [round((1 + delta) * guess, 5), round((1 - delta) * guess, 5)]
+    rhos: list = []
+    errs: list = []
+    hist: list = []
+    found = None
+    i, restarts = 0, 0
+    while True:
+        if i == 0:
+            (r0, e0), (r1, e1) = run_many(mu_s[:2])
+            rhos, errs = [r0 - target, r1 - target], [e0, e1]
+            hist += [(mu_s[0], r0, e0), (mu_s[1], r1, e1)]
+            if check_to_stop(rhos[0], e0):
+                found = mu_s[0]
+            if check_to_stop(rhos[1], e1):
+                found = mu_s[1]
+            try:
+                mu_s.append(round(new_mu(rhos, mu_s, 0), 5))
+            except GuessError:
+                restarts += 1
+                if restarts > 8:  # the reference would halve the guess forever
+                    raise
+                guess = change_guess(rhos, guess)
+                mu_s = [round((1 + delta) * guess, 5), round((1 - delta) * guess, 5)]
+                continue
+            i += 1
+        else:
+            (r, e), = run_many([mu_s[i]])
+            rhos.append(r - target)
+            errs.append(e)
+            hist.append((mu_s[i], r, e))
+            try:
+                mu_s.append(new_mu(rhos, mu_s, i - 1))
+            except JError:
+                break
+        if check_to_stop(rhos[-1], errs[-1]):
+            found = mu_s[len(rhos) - 1]
+        if found is not None or i == max_i:
+            break
+        i += 1
+    mu = found if found is not None else mu_s[len(rhos) - 1]
+    return mu, hist, found is not None
 
 
 def windowed_means(series, cut_fraction: float = 0.0, excluded: Sequence[int] = ()):
@@ -150,19 +274,3 @@ def windowed_means(series, cut_fraction: float = 0.0, excluded: Sequence[int] = 
         evol.append(float(np.mean(v)))
         err.append(float(np.var(v) ** 0.5 / (runs - len(excluded)) ** 0.5))
     return np.array(evol), np.array(err)
-
-
-def secant_mu_search(run_density: Callable[[float], float], mu0: float, mu1: float,
-                     target: float = 1.0, tol: float = 1e-3, max_iter: int = 15):
-    """Secant search of mu with density(mu) = target (the orchestration of
-    Secant_method_function, mylib.py:599-706, with each evaluation a server
-    run).  Returns (mu, history)."""
-    mus = [mu0, mu1]
-    res = [run_density(mu0) - target, run_density(mu1) - target]
-    for _ in range(max_iter):
-        if check_to_stop(res[-1], tol):
-            break
-        m = new_mu(res, mus)
-        mus.append(m)
-        res.append(run_density(m) - target)
-    return mus[-1], list(zip(mus, res))

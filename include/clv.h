@@ -35,6 +35,7 @@ extern "C" {
 #define CLV_ERR_BUDGET (-4)      /* request does not fit the HBM budget     */
 #define CLV_ERR_STATE (-5)       /* call not valid in the current state     */
 #define CLV_ERR_NOTFOUND (-6)    /* unknown request / chain id              */
+#define CLV_ERR_CANCELLED (-7)   /* run cancelled by clv_server_cancel      */
 
 #define CLV_FP64 0
 #define CLV_FP32 1
@@ -249,6 +250,8 @@ typedef struct clv_server_info {
   int64_t bytes_in_use, budget_bytes, pools;
   double site_updates;   /* site-updates executed so far                    */
   double busy_seconds;   /* wall time with at least one run in flight        */
+  int64_t cancelled;     /* runs ended by clv_server_cancel                  */
+  double pool_setup_seconds;  /* worker time spent creating slot pools      */
 } clv_server_info;
 
 typedef struct clv_server clv_server;
@@ -257,6 +260,8 @@ typedef struct clv_server clv_server;
 int clv_server_create(int32_t device, int32_t precision, int64_t hbm_budget_bytes,
                       clv_server **out);
 int clv_server_destroy(clv_server *srv);
+/* Stamps t_submit on entry, before any lock: time a client spends blocked in
+ * the call counts toward the run's latency.  Never waits for device work. */
 int clv_server_submit(clv_server *srv, const clv_request *req, int64_t *id);
 /* Non-blocking; out->status tells where the run is. */
 int clv_server_poll(clv_server *srv, int64_t id, clv_result *out);
@@ -265,6 +270,15 @@ int clv_server_wait(clv_server *srv, int64_t id, double timeout_s, clv_result *o
 /* Forgets a finished (done or failed) run's record, which the server keeps
  * for poll/wait until then; CLV_ERR_STATE while it is queued or running. */
 int clv_server_release(clv_server *srv, int64_t id);
+/* Cancels a queued or running run.  A queued run leaves the queue; a running
+ * run's slot is freed at its pool's next tick boundary (the other slots of the
+ * pool are unaffected) and its memory is available to new runs from then on.
+ * The run ends with status CLV_ERR_CANCELLED and the step statistics it had
+ * reached.  CLV_ERR_STATE if the run has already finished.  The reference has
+ * no in-process control of a run: it scans SLURM logs for failed runs and
+ * relaunches them as new array jobs (mylib.py:160-199); a client of this
+ * server cancels the run and submits it again. */
+int clv_server_cancel(clv_server *srv, int64_t id);
 int clv_server_info_get(clv_server *srv, clv_server_info *out);
 /* The server's monotonic clock, seconds. */
 double clv_server_clock(void);

@@ -702,3 +702,52 @@ def test_full_size_uneven_batch_properties():
             assert (bs.dt, bs.t_langevin, bs.steps) == (os_.dt, os_.t_langevin, os_.steps), i
             assert bo.count == oo.count == 2 and bo.n_tot == oo.n_tot and bo.energy == oo.energy, i
         assert base[i][1].nonfinite == 0
+
+
+def test_set_params_between_propagate_calls(oracle_lib):
+    """Parameters changed between two propagate calls take effect on the next
+    step in every schedule: the fused kernel's next-step |D_0| partials were
+    computed with the old coefficients, so they are recomputed.  Two-pass
+    eager, fused eager and fused graph replay give the same fields and step
+    statistics bit for bit, and the FP64 result matches the oracle run with
+    the same change."""
+    print("case: set_params between propagate calls", flush=True)
+    from cl_var_dt_amd._abi import SCHED_FUSED, SCHED_TWO_PASS
+    p1 = [_params(seed=120 + i, mu=0.6) for i in range(2)]
+    p2 = [_params(seed=120 + i, mu=1.4, G=0.15, c1=0.03) for i in range(2)]
+    for prec in (CLV_FP32, CLV_FP64):
+        shapes = [(64, 4, 1, 8), (32, 8, 1, 16)] if prec == CLV_FP32 else [(32, 2, 1, 8), (16, 4, 1, 6)]
+        runs = []
+        for sched, graphs in ((SCHED_TWO_PASS, False), (SCHED_FUSED, False), (SCHED_FUSED, True)):
+            e = _engine(shapes, prec)
+            for i in range(2):
+                e.set_params(i, p1[i])
+                e.init_fields(i, 1.0, 0.1)
+            e.set_schedule(sched)
+            e.set_graphs(graphs)
+            e.propagate(6, 3)
+            for i in range(2):
+                e.set_params(i, p2[i])
+            e.propagate(8, 3)
+            runs.append([(e.get_fields(i), e.stats(i), e.observables(i)[0]) for i in range(2)])
+            assert e.schedule(0) == sched
+            e.close()
+        ref = runs[0]
+        for k, other in enumerate(runs[1:]):
+            for i in range(2):
+                (fa, sa, oa), (fb, sb, ob) = ref[i], other[i]
+                assert np.array_equal(fa[0], fb[0]) and np.array_equal(fa[1], fb[1]), (prec, k, i)
+                assert (sa.dt, sa.max_drift, sa.mean_drift, sa.t_langevin) == \
+                       (sb.dt, sb.max_drift, sb.mean_drift, sb.t_langevin), (prec, k, i)
+                assert oa.n_tot == ob.n_tot and oa.energy == ob.energy, (prec, k, i)
+        if prec == CLV_FP64:
+            orc = oracle_lib.Oracle(Shape(*shapes[0]), p1[0], nthreads=4)
+            orc.init_fields(1.0, 0.1)
+            orc.propagate(6, 3)
+            orc.set_params(p2[0])
+            orc.propagate(8, 3)
+            c, d = orc.get_fields()
+            (a, b), st, _ = ref[0]
+            assert max(np.max(abs(a - c)), np.max(abs(b - d))) < 1e-12
+            assert st.dt == pytest.approx(orc.stats().dt, rel=1e-12)
+            orc.close()

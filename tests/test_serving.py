@@ -39,31 +39,201 @@ def test_secant_search_on_a_smooth_curve():
     assert len(hist) < 10
 
 
+def test_new_mu_guard_falls_back_to_older_points():
+    """new_mu (mylib.py:561-580): a step into [10 max(mu), 0] -- reachable only
+    when no mu is positive -- moves j to an older point; none left raises
+    GuessError on the first step and JError after it."""
+    # i = 1: points (mu1, r1) = (-1.0, 1.0), (mu2, r2) = (-2.0, 0.5): secant root -3.0
+    # lies in [10 max(mu), 0] = [-10, 0], so j = 0 is tried: (mu0, r0) = (-4.0, -1.0)
+    # with (mu2, r2) gives (0.5 * -4 + 1.0 * -2) / 1.5 = -8/3, inside too: JError
+    with pytest.raises(serving.JError):
+        serving.new_mu([-1.0, 1.0, 0.5], [-4.0, -1.0, -2.0], 1)
+    # with mu0 positive the guard is inert: the plain secant of the last pair
+    r, mu = [-1.0, 1.0, 0.5], [4.0, -1.0, -2.0]
+    assert serving.new_mu(r, mu, 1) == pytest.approx((0.5 * -1.0 - 1.0 * -2.0) / (0.5 - 1.0))
+    # first step into the refused range: GuessError
+    with pytest.raises(serving.GuessError):
+        serving.new_mu([1.0, 0.5], [-1.0, -2.0], 0)
+    # fallback that succeeds: the last pair's root (-0.4) is refused, the older pair's
+    # root (-1.3) lies below 10 max(mu) = -1 and is taken
+    r, mu = [0.3, 0.5, 0.25], [-0.1, -0.2, -0.3]
+    last = (r[2] * mu[1] - r[1] * mu[2]) / (r[2] - r[1])
+    older = (r[2] * mu[0] - r[0] * mu[2]) / (r[2] - r[0])
+    assert -1.0 <= last <= 0.0 and older < -1.0
+    assert serving.new_mu(r, mu, 1) == pytest.approx(older)
+
+
+def test_new_mu_degenerate_step_is_refused():
+    # equal residuals: no secant root; the older point is used, or an error raised
+    with pytest.raises(serving.GuessError):
+        serving.new_mu([0.3, 0.3], [0.5, 0.7], 0)
+    r, mu = [-0.2, 0.3, 0.3], [0.4, 0.5, 0.7]
+    assert serving.new_mu(r, mu, 1) == pytest.approx((0.3 * 0.4 + 0.2 * 0.7) / 0.5)
+
+
+def test_change_guess_and_secant_method_stop_on_the_error():
+    assert serving.change_guess([0.5, 0.2], 0.8) == 0.4
+    with pytest.raises(serving.GuessError):
+        serving.change_guess([0.5, -0.2], 0.8)
+    f = lambda mu: 0.5 + np.tanh(mu)          # noqa: E731  density(mu), monotone
+    calls = []
+
+    def run_many(mus):
+        calls.append(list(mus))
+        return [(f(m), 0.02) for m in mus]
+    mu, hist, ok = serving.secant_method(run_many, 0.8, 0.25)
+    assert ok and abs(f(mu) - 1.0) < 0.02
+    assert calls[0] == [1.0, 0.6] and all(len(c) == 1 for c in calls[1:])
+    # the residual error, not a fixed tolerance, ends the search
+    mu2, hist2, ok2 = serving.secant_method(lambda mus: [(f(m), 1e-9) for m in mus], 0.8, 0.25)
+    assert ok2 and len(hist2) > len(hist) and abs(f(mu2) - 1.0) < 1e-9
+    mu3, hist3 = serving.secant_mu_search(lambda m: (f(m), 0.02), 1.0, 0.6)
+    assert abs(hist3[-1][1]) < 0.02 and len(hist3) == len(hist)
+
+
 # ------------------------------------------------------------------ GPU ----
+def _wait_running(srv, rid, timeout=30.0):
+    t0 = time.perf_counter()
+    while srv.poll(rid).status != 1:          # CLV_RUN_RUNNING
+        assert time.perf_counter() - t0 < timeout, "run never started"
+        time.sleep(0.001)
+
+
 @pytest.mark.gpu
 def test_same_result_alone_and_in_the_mix():
+    """A run admitted while other runs are on the device (shown from the
+    server's own admit/done stamps) returns the same result, bit for bit, as
+    the run alone."""
     print("case: serving alone vs mix", flush=True)
     shape_a = Shape(64, 64, 1, 32)
     pa = _p(11, mu=0.9, nt=32)
     with serving.Server(precision=CLV_FP32) as srv:
         alone = srv.wait(srv.submit(shape_a, pa, nsteps=120, eval_every=10))
     with serving.Server(precision=CLV_FP32) as srv:
-        others = [(Shape(128, 128, 1, 32), _p(21, mu=1.2, nt=32), 200),
-                  (Shape(64, 64, 1, 64), _p(22, mu=0.6, nt=64), 90),
+        others = [(Shape(128, 128, 1, 32), _p(21, mu=1.2, nt=32), 6000),
+                  (Shape(64, 64, 1, 64), _p(22, mu=0.6, nt=64), 3000),
                   (Shape(64, 64, 1, 32), _p(23, mu=1.4, nt=32), 150)]
         ids = [srv.submit(s, p, nsteps=n, eval_every=10) for s, p, n in others[:2]]
-        time.sleep(0.05)
+        for i in ids:
+            _wait_running(srv, i)
         rid = srv.submit(shape_a, pa, nsteps=120, eval_every=10)     # arrives mid-flight
-        time.sleep(0.02)
         ids.append(srv.submit(*others[2][:2], nsteps=others[2][2], eval_every=10))
         mixed = srv.wait(rid)
         rest = [srv.wait(i) for i in ids]
         info = srv.info()
     assert info.completed == 4 and info.failed == 0
+    m = mixed.raw
+    # admitted while both earlier runs were in flight, and ended before they did
+    for r in rest[:2]:
+        assert r.raw.t_admit < m.t_admit < r.raw.t_done, (r.raw.t_admit, m.t_admit, r.raw.t_done)
+    assert m.t_done < min(r.raw.t_done for r in rest[:2])
     assert mixed.count == alone.count == 12
     for f in ("n_tot", "density", "Sz", "energy", "t_langevin", "steps"):
         assert getattr(mixed, f) == getattr(alone, f), f
     assert all(r.steps == n for r, (_, _, n) in zip(rest, [others[0], others[1], others[2]]))
+
+
+@pytest.mark.gpu
+def test_submit_does_not_wait_for_pool_creation():
+    """clv_server_submit never queues behind the worker's device work: while a
+    4 GB pool for a 256x256x1x128 run is being created and stepped, every
+    submit returns in under 1 ms, and t_submit is stamped on entry."""
+    print("case: serving submit latency", flush=True)
+    with serving.Server(precision=CLV_FP32) as srv:
+        t0 = srv.clock()
+        big = srv.submit(Shape(256, 256, 1, 128), _p(41, nt=128), nsteps=20, eval_every=10)
+        durations, stamps, ids = [], [], []
+        while srv.clock() - t0 < 0.3:
+            ts = srv.clock()
+            ids.append(srv.submit(Shape(16, 16, 1, 8), _p(42 + len(ids)), nsteps=2, eval_every=0))
+            te = srv.clock()
+            durations.append(te - ts)
+            stamps.append(ts - t0)
+            time.sleep(5e-4)
+        raw = srv.poll(ids[0])
+        assert ts - 1e-3 <= srv.poll(ids[-1]).t_submit <= te
+        assert stamps[0] <= raw.t_submit - t0 <= stamps[0] + durations[0]
+        srv.wait(big, timeout=120)
+        for i in ids:
+            srv.wait(i, timeout=120, release=True)
+        info = srv.info()
+    setup = info.pool_setup_seconds
+    during = sum(1 for t in stamps if t < setup)
+    print(f"{len(durations)} submits, max {1e3 * max(durations):.3f} ms, pool setup {1e3 * setup:.1f} ms, "
+          f"{during} submits during it", flush=True)
+    assert info.completed == len(ids) + 1 and info.failed == 0
+    assert max(durations) < 1e-3
+
+
+@pytest.mark.gpu
+def test_cancel_frees_the_slot_and_leaves_other_runs_unchanged():
+    """clv_server_cancel: a queued run never starts; a running run's slot is
+    freed at its pool's next tick boundary while the pool's other run goes on
+    to the same result as alone; the memory then held by no run is used by a
+    run of another shape that did not fit the budget before."""
+    print("case: serving cancel", flush=True)
+    from cl_var_dt_amd import _abi
+    a, b = Shape(64, 64, 1, 32), Shape(64, 64, 1, 64)
+    pa = _p(51, mu=0.9, nt=32)
+    n_keep = 20000
+    with serving.Server(precision=CLV_FP32) as srv:
+        alone = srv.wait(srv.submit(a, pa, nsteps=n_keep, eval_every=100), timeout=300, release=True)
+    budget = int(2.5 * 4 * a.sites * 4 * 8)   # one pool of two a-slots, no room for a b-pool
+    with serving.Server(precision=CLV_FP32, hbm_budget_bytes=budget) as srv:
+        long_run = srv.submit(a, _p(52, nt=32), nsteps=10 ** 7)
+        keep = srv.submit(a, pa, nsteps=n_keep, eval_every=100)
+        _wait_running(srv, long_run)
+        _wait_running(srv, keep)
+        queued = srv.submit(a, _p(53, nt=32), nsteps=10)   # no free slot, no room for a pool
+        other = srv.submit(b, _p(54, nt=64), nsteps=30, eval_every=10)
+        time.sleep(0.05)
+        assert srv.poll(queued).status == 0 and srv.poll(other).status == 0
+        srv.cancel(queued)
+        with pytest.raises(_abi.ClvError) as e:
+            srv.wait(queued, timeout=30)
+        assert e.value.status == _abi.CLV_ERR_CANCELLED and srv.poll(queued).t_admit == 0.0
+        srv.cancel(long_run)
+        with pytest.raises(_abi.ClvError) as e:
+            srv.wait(long_run, timeout=30)
+        assert e.value.status == _abi.CLV_ERR_CANCELLED
+        lr = srv.poll(long_run)
+        assert 0 < lr.stats.steps < 10 ** 7
+        # the pool still serves `keep`, so the other shape still has no room
+        assert srv.poll(keep).status == 1 and srv.poll(other).status == 0
+        kept = srv.wait(keep, timeout=300)
+        done = srv.wait(other, timeout=60)
+        assert done.steps == 30 and done.raw.t_admit >= max(lr.t_done, kept.raw.t_done)
+        with pytest.raises(_abi.ClvError) as e:
+            srv.cancel(keep)                           # finished: nothing to cancel
+        assert e.value.status == _abi.CLV_ERR_STATE
+        info = srv.info()
+        assert info.cancelled == 2 and info.completed == 2 and info.failed == 0
+        assert info.bytes_in_use <= budget
+    for f in ("n_tot", "density", "Sz", "energy", "t_langevin", "steps"):
+        assert getattr(kept, f) == getattr(alone, f), f
+
+
+@pytest.mark.gpu
+def test_device_allocation_failure_fails_the_run():
+    """A run that fits the budget but not the device memory left (taken by
+    another allocation after the server started), with nothing else in
+    flight, fails with CLV_ERR_NOMEM and a message instead of waiting
+    forever; once the memory is back the same request runs."""
+    print("case: serving allocation failure", flush=True)
+    from cl_var_dt_amd import _abi
+    from tests import hip_rt
+    big = Shape(256, 256, 1, 128)                   # a pool of 3 slots, 3.3 GB
+    with serving.Server(precision=CLV_FP32, hbm_budget_bytes=16 << 30) as srv:
+        free, _ = hip_rt.mem_get_info()
+        hog = hip_rt.DeviceBuffer(free - (1 << 30))
+        rid = srv.submit(big, _p(61, nt=128), nsteps=1)
+        with pytest.raises(_abi.ClvError) as e:
+            srv.wait(rid, timeout=120)
+        assert e.value.status == _abi.CLV_ERR_NOMEM and "hipMalloc" in str(e.value)
+        hog.close()
+        assert srv.wait(srv.submit(big, _p(61, nt=128), nsteps=1), timeout=120).steps == 1
+        info = srv.info()
+        assert info.failed == 1 and info.completed == 1
 
 
 @pytest.mark.gpu
