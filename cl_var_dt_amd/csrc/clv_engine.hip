@@ -19,6 +19,7 @@
 #include <fstream>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <sstream>
 #include <string>
 #include <vector>
@@ -1193,15 +1194,19 @@ int setup_bucket(clv_ctx *ctx, Bucket &b) {
   if ((rc = dalloc(ctx, b, reinterpret_cast<void **>(&b.d_acc), sizeof(double) * 5 * b.nch))) return rc;
   if ((rc = dalloc(ctx, b, reinterpret_cast<void **>(&b.d_count), sizeof(int64_t) * b.nch))) return rc;
   if ((rc = upload_row_tables(ctx, b.X, b.d_tw, b.d_p2))) return rc;
-  HIPCHK(ctx, hipMemset(b.d_st, 0, 2 * sizeof(ChainState) * b.nch));
-  HIPCHK(ctx, hipMemset(b.d_prm, 0, sizeof(ChainParams) * b.nch));
-  HIPCHK(ctx, hipMemset(b.d_F, 0, sizeof(double) * 3 * b.nch * b.Vs));
-  HIPCHK(ctx, hipMemset(b.d_acc, 0, sizeof(double) * 5 * b.nch));
-  HIPCHK(ctx, hipMemset(b.d_count, 0, sizeof(int64_t) * b.nch));
+  // Zeroed on the context's own stream and waited for: a null-stream memset
+  // is not ordered with the context's non-blocking stream, so a field
+  // initialisation launched right after could run first and be overwritten.
+  HIPCHK(ctx, hipMemsetAsync(b.d_st, 0, 2 * sizeof(ChainState) * b.nch, ctx->stream));
+  HIPCHK(ctx, hipMemsetAsync(b.d_prm, 0, sizeof(ChainParams) * b.nch, ctx->stream));
+  HIPCHK(ctx, hipMemsetAsync(b.d_F, 0, sizeof(double) * 3 * b.nch * b.Vs, ctx->stream));
+  HIPCHK(ctx, hipMemsetAsync(b.d_acc, 0, sizeof(double) * 5 * b.nch, ctx->stream));
+  HIPCHK(ctx, hipMemsetAsync(b.d_count, 0, sizeof(int64_t) * b.nch, ctx->stream));
   for (int k = 0; k < 2; ++k) {
-    HIPCHK(ctx, hipMemset(b.psi[k], 0, field));
-    HIPCHK(ctx, hipMemset(b.psib[k], 0, field));
+    HIPCHK(ctx, hipMemsetAsync(b.psi[k], 0, field, ctx->stream));
+    HIPCHK(ctx, hipMemsetAsync(b.psib[k], 0, field, ctx->stream));
   }
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   b.host_steps.assign(b.nch, 0);
   b.host_eval.assign(b.nch, 0);
   b.host_eval_from.assign(b.nch, 0);
