@@ -2202,3 +2202,166 @@ int clv_get_schedule(const clv_ctx *ctx, int32_t chain) {
 }
 
 }  // extern "C"
+
+// ====================================================== allocation-free step ==
+// clv_step_noalloc: one step of one chain in caller-owned device buffers with
+// no device allocation.  Everything its kernels need beyond the fields -- the
+// chain's parameter record and controller state, the row-transform tables and
+// the per-(tau, tile) (max, sum) partials of |D_0| -- is carved out of
+// `scratch`, a file-scope device array of kScratchBytes that every GPU the
+// library runs on holds until the process ends.  Host code never passes the
+// array's name to a kernel or a copy: it looks up each device's address of
+// the array once (hipGetSymbolAddress, with that device current) and uses
+// that pointer.
+constexpr size_t kScratchBytes = size_t(1) << 20;
+static __device__ __align__(256) char scratch[kScratchBytes];
+
+namespace {
+
+std::mutex g_scratch_mu;
+std::vector<void *> g_scratch_ptr;  // device address of `scratch`, by device index
+
+// This device's address of `scratch` (the device is current), looked up on the
+// first call on the device.
+int scratch_address(int device, void **out) {
+  std::lock_guard<std::mutex> g(g_scratch_mu);
+  if (int(g_scratch_ptr.size()) <= device) g_scratch_ptr.resize(size_t(device) + 1, nullptr);
+  if (!g_scratch_ptr[device]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      (void)hipGetLastError();
+      return fail(nullptr, CLV_ERR_HIP, std::string("clv_step_noalloc: hipGetSymbolAddress(scratch): ") +
+                                            hipGetErrorString(e));
+    }
+    g_scratch_ptr[device] = p;
+  }
+  *out = g_scratch_ptr[device];
+  return CLV_OK;
+}
+
+size_t align256(size_t n) { return (n + 255) & ~size_t(255); }
+
+bool overlaps(const void *a, const void *b, size_t n) {
+  const char *x = static_cast<const char *>(a), *y = static_cast<const char *>(b);
+  return x < y + n && y < x + n;
+}
+
+}  // namespace
+
+extern "C" int clv_step_noalloc(const clv_shape *shape, int32_t precision, const clv_params *p,
+                                const void *psi, const void *psibar, void *psi_out, void *psibar_out,
+                                clv_step_stats *stats, int32_t device, void *stream) {
+#define NOALLOC_CHK(expr)                                                                          \
+  do {                                                                                             \
+    const hipError_t e_ = (expr);                                                                  \
+    if (e_ != hipSuccess) {                                                                        \
+      (void)hipGetLastError();                                                                     \
+      (void)hipStreamSynchronize(static_cast<hipStream_t>(stream)); /* host records in use */     \
+      return fail(nullptr, CLV_ERR_HIP, std::string("clv_step_noalloc: ") + #expr + ": " +         \
+                                            hipGetErrorString(e_));                                \
+    }                                                                                              \
+  } while (0)
+  if (!shape || !p || !psi || !psibar || !psi_out || !psibar_out || !stats)
+    return fail(nullptr, CLV_ERR_ARG, "clv_step_noalloc: NULL argument");
+  if (precision != CLV_FP32 && precision != CLV_FP64)
+    return fail(nullptr, CLV_ERR_ARG, "clv_step_noalloc: precision must be CLV_FP32 or CLV_FP64");
+  const clv_shape &s = *shape;
+  if (s.nx <= 0 || s.ny <= 0 || s.nz <= 0 || s.nt < 2 || !supported_x(s.nx) ||
+      int64_t(s.nx) * s.ny * s.nz * s.nt >= (int64_t(1) << 31))
+    return fail(nullptr, CLV_ERR_ARG, "clv_step_noalloc: bad or unsupported shape");
+  if (p->kin_dims != 1)
+    return fail(nullptr, CLV_ERR_ARG, "clv_step_noalloc: kin_dims must be 1 (the transverse term needs a buffer)");
+  if (p->conj_drift_mode != CLV_CONJ_DRIFT_REFERENCE && p->conj_drift_mode != CLV_CONJ_DRIFT_ACTION)
+    return fail(nullptr, CLV_ERR_ARG, "clv_step_noalloc: bad conj_drift_mode");
+  if (!(p->a_x > 0.0) || !(p->DT > 0.0) || !(p->eps > 0.0))
+    return fail(nullptr, CLV_ERR_ARG, "clv_step_noalloc: a_x, DT and eps must be positive");
+  if (stats->steps < 0) return fail(nullptr, CLV_ERR_ARG, "clv_step_noalloc: stats->steps < 0");
+  Bucket b;
+  b.shape = s;
+  b.nch = 1;
+  plan_bucket(b, precision);
+  // the scratch carve-up: parameters, two state slots, tables, partials
+  size_t off = 0;
+  auto take = [&](size_t n) {
+    const size_t o = off;
+    off = align256(off + n);
+    return o;
+  };
+  const size_t o_prm = take(sizeof(ChainParams));
+  const size_t o_st = take(2 * sizeof(ChainState));
+  const size_t o_tw = take(2 * b.esz * b.X);
+  const size_t o_p2 = take(sizeof(double) * b.X);
+  const size_t o_part = take(sizeof(double) * b.part_slot);
+  if (off > kScratchBytes)
+    return fail(nullptr, CLV_ERR_ARG,
+                "clv_step_noalloc: shape " + std::to_string(s.nx) + "x" + std::to_string(s.ny) + "x" +
+                    std::to_string(s.nz) + "x" + std::to_string(s.nt) + " needs " + std::to_string(off) +
+                    " bytes of scratch, more than its " + std::to_string(kScratchBytes) +
+                    " (Nt x row tiles x 16 bytes of partials)");
+  const size_t fbytes = size_t(b.S) * 4 * b.esz;
+  const void *outs[2] = {psi_out, psibar_out};
+  for (const void *o : outs)
+    if (overlaps(o, psi, fbytes) || overlaps(o, psibar, fbytes))
+      return fail(nullptr, CLV_ERR_ARG, "clv_step_noalloc: output fields overlap the input fields");
+  if (overlaps(psi_out, psibar_out, fbytes))
+    return fail(nullptr, CLV_ERR_ARG, "clv_step_noalloc: psi_out and psibar_out overlap");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) {
+    (void)hipGetLastError();
+    return fail(nullptr, CLV_ERR_ARG, "clv_step_noalloc: bad device index");
+  }
+  NOALLOC_CHK(hipSetDevice(device));
+  void *scr = nullptr;
+  if (int rc = scratch_address(device, &scr)) return rc;
+  char *base = static_cast<char *>(scr);
+  b.d_prm = reinterpret_cast<ChainParams *>(base + o_prm);
+  b.d_st = reinterpret_cast<ChainState *>(base + o_st);
+  b.d_tw = base + o_tw;
+  b.d_p2 = reinterpret_cast<double *>(base + o_p2);
+  b.d_part = reinterpret_cast<double *>(base + o_part);
+  b.psi[0] = const_cast<void *>(psi);
+  b.psib[0] = const_cast<void *>(psibar);
+  b.psi[1] = psi_out;
+  b.psib[1] = psibar_out;
+  b.schedule = 1;
+  b.epoch = 0;
+  // host records, kept alive until the stream is synchronized below
+  const ChainParams hp = to_device_params(*p, 0, 1);
+  ChainState st;
+  std::memset(&st, 0, sizeof(st));
+  st.dt = stats->dt;
+  st.max_drift = stats->max_drift;
+  st.mean_drift = stats->mean_drift;
+  st.t_langevin = stats->t_langevin;
+  st.steps = stats->steps;
+  st.cur_step = stats->steps > 0 ? stats->steps - 1 : 0;
+  st.clamp_low = stats->clamp_low;
+  st.clamp_high = stats->clamp_high;
+  st.nonfinite = stats->nonfinite;
+  st.parity = 0;  // the inputs are buffer 0, the outputs buffer 1
+  std::vector<char> tw;
+  std::vector<double> p2;
+  row_tables_host(b.X, precision, &tw, &p2);
+  clv_ctx ctx;  // launch record only: no device memory, the caller's stream
+  ctx.device = device;
+  ctx.precision = precision;
+  ctx.stream = static_cast<hipStream_t>(stream);
+  ctx.use_graphs = false;
+  NOALLOC_CHK(hipMemcpyAsync(b.d_prm, &hp, sizeof(hp), hipMemcpyHostToDevice, ctx.stream));
+  NOALLOC_CHK(hipMemcpyAsync(b.d_st, &st, sizeof(st), hipMemcpyHostToDevice, ctx.stream));
+  NOALLOC_CHK(hipMemcpyAsync(b.d_tw, tw.data(), tw.size(), hipMemcpyHostToDevice, ctx.stream));
+  NOALLOC_CHK(hipMemcpyAsync(b.d_p2, p2.data(), sizeof(double) * p2.size(), hipMemcpyHostToDevice, ctx.stream));
+  // the two-pass step of clv_propagate: partials, device dt, update into buffer 1
+  row_kernel(&ctx, b, 0, 0, 1, 0);
+  controller(&ctx, b, 0, 1, 0, 0.0, 0);
+  row_kernel(&ctx, b, 1, 0, 1, 0);
+  NOALLOC_CHK(hipGetLastError());
+  ChainState out;
+  NOALLOC_CHK(hipMemcpyAsync(&out, b.d_st, sizeof(out), hipMemcpyDeviceToHost, ctx.stream));
+  NOALLOC_CHK(hipStreamSynchronize(ctx.stream));
+  ctx.stream = nullptr;  // the caller's
+  stats_from_state(out, stats);
+  return CLV_OK;
+#undef NOALLOC_CHK
+}

@@ -213,6 +213,33 @@ int clv_set_graphs(clv_ctx *ctx, int32_t on);
 int clv_set_profiling(clv_ctx *ctx, int32_t on);
 int clv_kernel_time(clv_ctx *ctx, int32_t kind, double *total_ms, int64_t *launches);
 
+/* ---- allocation-free step ------------------------------------------------
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory -- code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc: one Langevin step of one chain held in caller-owned device
+ * buffers, the step of clv_propagate (device-resident dt, the two-pass kernels,
+ * identical results) without observables, calling no hipMalloc or hipFree.
+ *   psi, psibar          device, the chain's fields in the library's internal
+ *                        layout: complex64 (CLV_FP32) or complex128 (CLV_FP64)
+ *                        value (site, c) at site*4 + c, site = (tau*Y*Z + y*Z + z)*X + x;
+ *   psi_out, psibar_out  device, the fields after the step (same layout; must
+ *                        not overlap the inputs);
+ *   stats                host, in: the chain's controller state (steps is the
+ *                        noise counter; all zero for a fresh chain); out: the
+ *                        state after the step;
+ *   stream               hipStream_t to run on (NULL: the default stream); the
+ *                        call synchronizes it before returning.
+ * The chain's parameter record, controller state, row-transform tables and its
+ * per-(tau, row tile) |D_0| partials live in a static 1 MiB device array that
+ * each GPU holds until the process ends; a shape whose Nt x row tiles x 16
+ * bytes of partials does not fit is rejected with CLV_ERR_ARG (256x256x1x128
+ * fits, 256x512x1x256 does not).  Every call on one GPU shares that array:
+ * calls that overlap on one GPU race.  kin_dims must be 1.  Errors:
+ * clv_last_error(NULL). */
+int clv_step_noalloc(const clv_shape *shape, int32_t precision, const clv_params *params,
+                     const void *psi, const void *psibar, void *psi_out, void *psibar_out,
+                     clv_step_stats *stats, int32_t device, void *stream);
+
 /* ---- serving: continuous batching of Langevin runs -----------------------
  * No counterpart in the reference, which runs one chain per process and
  * batches runs as SLURM job arrays polled from Python (mylib.py:511-543,
