@@ -220,9 +220,10 @@ def run_serving(args, log):
         log(f"serving: capacity {capacity_su:.3e} SU/s, Poisson rate {rate:.2f} req/s")
 
         # secant clients on a serving shape, issuing follow-up runs while the
-        # stream runs: each mu is a batch of independent runs whose mean density
-        # and standard error feed the reference's secant loop (stop when the
-        # residual is inside the error)
+        # stream runs: each mu is a batch of independent runs (n0 = 0.25 start,
+        # DT = 1e-2, densities averaged over the second half of each run) whose
+        # mean density and standard error feed the reference's secant loop
+        # (stop when the residual is inside the error)
         secant_out = []
 
         def secant_client(k):
@@ -232,9 +233,10 @@ def run_serving(args, log):
                 ids = {}
                 for m in mus:
                     ids[m] = [srv.submit(shape, default_params(
-                        mu=m, G=0.1, c1=-0.05, Q=0.1, eps=1.0 / 32, DT=1e-3,
+                        mu=m, G=0.1, c1=-0.05, Q=0.1, eps=1.0 / 32, DT=1e-2,
                         seed=10_000 * k + 100 * len(secant_calls) + j),
-                        nsteps=args.secant_steps, eval_every=10, eval_from=args.secant_steps // 4)
+                        nsteps=args.secant_steps, eval_every=10, eval_from=args.secant_steps // 2,
+                        n0=0.25, var=0.05)
                         for j in range(args.secant_runs)]
                 secant_calls.append(list(mus))
                 out = []
@@ -244,7 +246,7 @@ def run_serving(args, log):
                 return out
 
             secant_calls = []
-            mu, hist, ok = serving.secant_method(run_many, args.secant_guess, 0.25, target=1.0,
+            mu, hist, ok = serving.secant_method(run_many, args.secant_guess, 0.5, target=1.0,
                                                  max_i=8)
             secant_out.append({"mu": mu, "converged": ok, "iterations": len(hist),
                                "density": hist[-1][1], "err": hist[-1][2],
@@ -343,10 +345,10 @@ def main() -> int:
     ap.add_argument("--serve-burst", type=int, default=24)
     ap.add_argument("--serve-seconds", type=float, default=15.0)
     ap.add_argument("--serve-load", type=float, default=0.7)
-    ap.add_argument("--secant-steps", type=int, default=2000)
+    ap.add_argument("--secant-steps", type=int, default=4000)
     ap.add_argument("--secant-runs", type=int, default=4, help="independent runs per secant mu")
     ap.add_argument("--secant-l", type=int, default=64, help="secant client lattice L (L x L x 1 x 32)")
-    ap.add_argument("--secant-guess", type=float, default=0.5)
+    ap.add_argument("--secant-guess", type=float, default=0.1)
     ap.add_argument("--prof-steps", type=int, default=10,
                     help="steps of the separate pass with per-kernel HIP events")
     ap.add_argument("--no-serving", action="store_true")

@@ -244,7 +244,10 @@ __device__ __forceinline__ void dft(C2<T> *x) {
   }
 }
 
-// Row length X = N1 * N2 of the two-level transform.
+// Row length X = N1 * N2 of the two-level transform: the default split per X
+// (RowFFT takes any N1 dividing X with N1 and X / N1 in the small DFTs above;
+// the engine instantiates the default and one alternative per length, and
+// its autotuner picks between them).
 template <int X> struct Split;
 template <> struct Split<4>   { static constexpr int N1 = 2, N2 = 2; };
 template <> struct Split<8>   { static constexpr int N1 = 4, N2 = 2; };
@@ -276,9 +279,10 @@ __host__ __device__ constexpr int row_pad(int X, int N2) {
   return p;
 }
 
-template <typename T, int X>
+template <typename T, int X, int N1_ = Split<X>::N1>
 struct RowFFT {
-  static constexpr int N1 = Split<X>::N1, N2 = Split<X>::N2;
+  static constexpr int N1 = N1_, N2 = X / N1_;
+  static_assert(N1 * N2 == X, "N1 must divide the row length");
   static constexpr int PAD = row_pad(X, N2);  // LDS cells per line
   __device__ static __forceinline__ int pos(int p) { return p + p / N2; }
 

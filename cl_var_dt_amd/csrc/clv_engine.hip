@@ -40,7 +40,7 @@ constexpr int kFusedWavesPerEU = 3;
 // D_0 at every site of TC tau-slices of `rows` rows; writes each slice's tile
 // (max |D_0|, sum |D_0|) -- compute_max_and_mean_drift, Propagator_without_cutoff.cu:231-265,
 // in FP64 and without atomics (tile_max_sum, one wave per slice).
-template <typename T, int X>
+template <typename T, int X, int N1>
 __global__ __launch_bounds__(256) void k_drift_stats(BucketArgs<T> A) {
   extern __shared__ __align__(16) char smem_raw[];
   const int chain = A.chain0 + blockIdx.z;
@@ -51,9 +51,9 @@ __global__ __launch_bounds__(256) void k_drift_stats(BucketArgs<T> A) {
   const int j = tid / X, x = tid - (tid / X) * X;
   const bool has_site = j < A.rows;
   const int rows = A.rows, TC = A.tcs;
-  Smem<T, X> sm(smem_raw, TC * rows);
+  Smem<T, X, N1> sm(smem_raw, TC * rows);
   double *vb = reinterpret_cast<double *>(sm.red_end());  // [TC][nthr] |D_0|
-  load_tables<T, X>(sm, A.tw, A.p2, drift_table_scale(P), tid, nthr);
+  load_tables<T, X, N1>(sm, A.tw, A.p2, drift_table_scale(P), tid, nthr);
   const int par = A.st[chain].parity;  // current buffer
   const C2<T> *psi = (par ? A.buf1_psi : A.buf0_psi) + chain * A.chain_stride;
   const C2<T> *psib = (par ? A.buf1_psib : A.buf0_psib) + chain * A.chain_stride;
@@ -73,7 +73,7 @@ __global__ __launch_bounds__(256) void k_drift_stats(BucketArgs<T> A) {
     }
   }
   __syncthreads();
-  RowFFT<T, X>::kinetic(sm.buf, sm.tw, sm.mult, TC * rows, tid, nthr);
+  RowFFT<T, X, N1>::kinetic(sm.buf, sm.tw, sm.mult, TC * rows, tid, nthr);
   const Coef<T> k = coef<T>(P);
 #pragma unroll
   for (int q = 0; q < TCMAX; ++q) {
@@ -155,7 +155,7 @@ __global__ __launch_bounds__(64) void k_controller(BucketArgs<T> A, int mode, do
 // The update of one tau-slice of `rows` rows: multiply_by_timestep + add_noise
 // + fields += drift (Propagator_without_cutoff.cu:131-157, 432-441) with the
 // drift recomputed in registers and the noise drawn from Philox in registers.
-template <typename T, int X>
+template <typename T, int X, int N1>
 __global__ __launch_bounds__(256, sizeof(T) == 4 ? 5 : 2) void k_update(BucketArgs<T> A) {
   extern __shared__ __align__(16) char smem_raw[];
   const int chain = A.chain0 + blockIdx.z;
@@ -165,8 +165,8 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? 5 : 2) void k_update(BucketAr
   const int tid = threadIdx.x, nthr = blockDim.x;
   const int j = tid / X, x = tid - (tid / X) * X;
   const bool has_site = j < A.rows;
-  Smem<T, X> sm(smem_raw, 8 * A.rows);
-  load_tables<T, X>(sm, A.tw, A.p2, drift_table_scale(P), tid, nthr);
+  Smem<T, X, N1> sm(smem_raw, 8 * A.rows);
+  load_tables<T, X, N1>(sm, A.tw, A.p2, drift_table_scale(P), tid, nthr);
   const ChainState &st = A.st[chain];
   const int par = st.in_parity;
   const double dt = st.dt;
@@ -193,7 +193,7 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? 5 : 2) void k_update(BucketAr
     }
   }
   __syncthreads();
-  RowFFT<T, X>::kinetic(sm.buf, sm.tw, sm.mult, 8 * rows, tid, nthr);
+  RowFFT<T, X, N1>::kinetic(sm.buf, sm.tw, sm.mult, 8 * rows, tid, nthr);
   if (!has_site) return;
 #pragma unroll
   for (int c = 0; c < 4; ++c) {
@@ -246,7 +246,7 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? 5 : 2) void k_update(BucketAr
 // tau = Nt-1 are still the old tau = 0.  Reads and writes the state once per
 // step; results are bit-identical to the two-pass schedule for every fbatch
 // and line-set count (the same per-line transforms and drift0_site split).
-template <typename T, int X>
+template <typename T, int X, int N1>
 __global__ __launch_bounds__(256, sizeof(T) == 4 ? kFusedWavesPerEU : 2) void k_fused(BucketArgs<T> A) {
   extern __shared__ __align__(16) char smem_raw[];
   const int chain = A.chain0 + blockIdx.y;
@@ -262,11 +262,11 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? kFusedWavesPerEU : 2) void k_
   // line sets of 8 + B row sets: 8 for the old fields' transforms, B slots for
   // the parked next-step K_0 inputs (used in the set of the batch taus)
   const int setw = (8 + B) * rows;
-  Smem<T, X> sm(smem_raw, A.fsets * setw);
+  Smem<T, X, N1> sm(smem_raw, A.fsets * setw);
   C2<T> *lpark = sm.red_end();  // [B-1][nthr] parked non-kinetic D_0 parts
   double *vb = reinterpret_cast<double *>(lpark + (B - 1) * nthr);  // [max(B,2)][nthr] |D_0|
-  using RF = RowFFT<T, X>;
-  load_tables<T, X>(sm, A.tw, A.p2, drift_table_scale(P), tid, nthr);
+  using RF = RowFFT<T, X, N1>;
+  load_tables<T, X, N1>(sm, A.tw, A.p2, drift_table_scale(P), tid, nthr);
   if (tid < 64) {
     const int n = Nt * A.ntiles;
     double m, s;
@@ -476,7 +476,7 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? kFusedWavesPerEU : 2) void k_
 // transform levels that one slice of a long row leaves mostly idle, with half
 // the barriers per slice).  Each thread still adds its slices in tau order,
 // so S does not change the result.
-template <typename T, int X>
+template <typename T, int X, int N1>
 __global__ __launch_bounds__(256) void k_obs(BucketArgs<T> A) {
   constexpr int S = obs_slices(int(sizeof(C2<T>)), X);
   extern __shared__ __align__(16) char smem_raw[];
@@ -490,8 +490,8 @@ __global__ __launch_bounds__(256) void k_obs(BucketArgs<T> A) {
   const int j = tid / X, x = tid - (tid / X) * X;
   const bool has_site = j < A.rows;
   const int rows = A.rows, Nt = A.Nt, tc = A.tc;
-  Smem<T, X> sm(smem_raw, 4 * S * rows);
-  load_tables<T, X>(sm, A.tw, A.p2, 1.0 / P.a_x, tid, nthr);
+  Smem<T, X, N1> sm(smem_raw, 4 * S * rows);
+  load_tables<T, X, N1>(sm, A.tw, A.p2, 1.0 / P.a_x, tid, nthr);
   const int par = st.parity;
   const C2<T> *psi = (par ? A.buf1_psi : A.buf0_psi) + chain * A.chain_stride;
   const C2<T> *psib = (par ? A.buf1_psib : A.buf0_psib) + chain * A.chain_stride;
@@ -530,7 +530,7 @@ __global__ __launch_bounds__(256) void k_obs(BucketArgs<T> A) {
       if (tt + S < tc) load_pair(t + S, tc - tt - S < S ? tc - tt - S : S, qn, bn);
     }
     __syncthreads();
-    RowFFT<T, X>::kinetic(sm.buf, sm.tw, sm.mult, 4 * cnt * rows, tid, nthr);
+    RowFFT<T, X, N1>::kinetic(sm.buf, sm.tw, sm.mult, 4 * cnt * rows, tid, nthr);
     if (has_site) {
 #pragma unroll
       for (int u = 0; u < S; ++u) {
@@ -777,25 +777,25 @@ BucketArgs<T> make_args(const Bucket &b, int chain0, int eval_every, bool fused 
 
 // ---- launchers, dispatched on X at run time ----
 // which: 0 drift_stats, 1 update, 2 obs, 3 fused
-template <typename T, int X>
+template <typename T, int X, int N1>
 void launch_row_kernels(const Bucket &b, hipStream_t s, int which, int chain0, int nch,
                         int eval_every) {
   const dim3 block(b.nthr);
   if (which == 0) {
     BucketArgs<T> A = make_args<T>(b, chain0, eval_every);
-    const size_t sh = Smem<T, X>::bytes(b.tcs * b.rows) + sizeof(double) * b.tcs * b.nthr;
-    hipLaunchKernelGGL((k_drift_stats<T, X>), dim3(b.ntiles, b.Nt / b.tcs, nch), block, sh, s, A);
+    const size_t sh = Smem<T, X, N1>::bytes(b.tcs * b.rows) + sizeof(double) * b.tcs * b.nthr;
+    hipLaunchKernelGGL((k_drift_stats<T, X, N1>), dim3(b.ntiles, b.Nt / b.tcs, nch), block, sh, s, A);
   } else if (which == 1) {
     BucketArgs<T> A = make_args<T>(b, chain0, eval_every);
-    const size_t sh = Smem<T, X>::bytes(8 * b.rows);
-    hipLaunchKernelGGL((k_update<T, X>), dim3(b.ntiles, b.Nt, nch), block, sh, s, A);
+    const size_t sh = Smem<T, X, N1>::bytes(8 * b.rows);
+    hipLaunchKernelGGL((k_update<T, X, N1>), dim3(b.ntiles, b.Nt, nch), block, sh, s, A);
   } else if (which == 2) {
     BucketArgs<T> A = make_args<T>(b, chain0, eval_every);
-    const size_t sh = Smem<T, X>::bytes(4 * obs_slices(int(sizeof(C2<T>)), X) * b.rows);
-    hipLaunchKernelGGL((k_obs<T, X>), dim3(b.ntiles, b.nchunks, nch), block, sh, s, A);
+    const size_t sh = Smem<T, X, N1>::bytes(4 * obs_slices(int(sizeof(C2<T>)), X) * b.rows);
+    hipLaunchKernelGGL((k_obs<T, X, N1>), dim3(b.ntiles, b.nchunks, nch), block, sh, s, A);
   } else {
     BucketArgs<T> A = make_args<T>(b, chain0, eval_every, true);
-    hipLaunchKernelGGL((k_fused<T, X>), dim3(b.ntiles, nch), block, b.fused_lds, s, A);
+    hipLaunchKernelGGL((k_fused<T, X, N1>), dim3(b.ntiles, nch), block, b.fused_lds, s, A);
   }
 }
 
@@ -846,10 +846,37 @@ void dispatch_transverse(const Bucket &b, hipStream_t s, int chain0, int nch, in
   }
 }
 
+// The row-transform splits instantiated: (X, N1) with X = N1 * N2, the
+// default of Split<X> first, then one alternative per length where the small
+// DFTs allow it.
+#define CLV_SWITCH_SPLIT(X_, N1_, CALL)                              \
+  switch ((X_) * 1000 + (N1_)) {                                    \
+    case 4002: CALL(4, 2); break;                                   \
+    case 8004: CALL(8, 4); break;                                   \
+    case 16004: CALL(16, 4); break;                                 \
+    case 16008: CALL(16, 8); break;                                 \
+    case 24006: CALL(24, 6); break;                                 \
+    case 24004: CALL(24, 4); break;                                 \
+    case 32008: CALL(32, 8); break;                                 \
+    case 32004: CALL(32, 4); break;                                 \
+    case 48008: CALL(48, 8); break;                                 \
+    case 48006: CALL(48, 6); break;                                 \
+    case 64008: CALL(64, 8); break;                                 \
+    case 64016: CALL(64, 16); break;                                \
+    case 96012: CALL(96, 12); break;                                \
+    case 96008: CALL(96, 8); break;                                 \
+    case 128016: CALL(128, 16); break;                              \
+    case 128008: CALL(128, 8); break;                               \
+    case 192016: CALL(192, 16); break;                              \
+    case 192012: CALL(192, 12); break;                              \
+    case 256016: CALL(256, 16); break;                              \
+    default: break;                                                 \
+  }
+
 template <typename T>
 void dispatch_row(const Bucket &b, hipStream_t s, int which, int chain0, int nch, int ev) {
-#define CLV_ROW(XX) launch_row_kernels<T, XX>(b, s, which, chain0, nch, ev)
-  CLV_SWITCH_X(b.X, CLV_ROW)
+#define CLV_ROW(XX, NN) launch_row_kernels<T, XX, NN>(b, s, which, chain0, nch, ev)
+  CLV_SWITCH_SPLIT(b.X, b.n1, CLV_ROW)
 #undef CLV_ROW
 }
 
@@ -942,8 +969,8 @@ void obs_finish(clv_ctx *c, const Bucket &b, int chain0, int nch, int ev) {
   });
 }
 
-// N2 of the two-level row transform (Split<X> in clv_device.h)
-int split_n2(int X) {
+// N2 of the default split of the two-level row transform (Split<X> in clv_device.h)
+int default_n2(int X) {
   switch (X) {
     case 4: case 8: return 2;
     case 16: case 24: case 32: return 4;
@@ -953,15 +980,41 @@ int split_n2(int X) {
     default: return 16;
   }
 }
+int default_n1(int X) { return X / default_n2(X); }
+
+// Splits N1 of length X instantiated in CLV_SWITCH_SPLIT, the default first.
+std::vector<int> split_options(int X) {
+  std::vector<int> o = {default_n1(X)};
+  switch (X) {
+    case 16: o.push_back(8); break;
+    case 24: o.push_back(4); break;
+    case 32: o.push_back(4); break;
+    case 48: o.push_back(6); break;
+    case 64: o.push_back(16); break;
+    case 96: o.push_back(8); break;
+    case 128: o.push_back(8); break;
+    case 192: o.push_back(12); break;
+    default: break;
+  }
+  return o;
+}
 
 // LDS bytes of the fused kernel with `sets` line sets of (8 + batch) row sets
 // each, plus the parked non-kinetic D_0 parts of batch - 1 taus per thread
 // and |D_0| of max(batch, 2) taus per thread for the partials.
-size_t fused_lds_bytes(const Bucket &b, int sets, int batch) {
+size_t fused_lds_bytes(const Bucket &b, int sets, int batch, int n1) {
   return size_t(b.esz) * 2 * b.X + size_t(b.esz / 2) * (b.X + (b.X & 1)) +
-         size_t(b.esz) * (8 + batch) * sets * b.rows * row_pad(b.X, split_n2(b.X)) +
+         size_t(b.esz) * (8 + batch) * sets * b.rows * row_pad(b.X, b.X / n1) +
          sizeof(double) * 64 + size_t(b.esz) * (batch - 1) * b.nthr +
          sizeof(double) * (batch < 2 ? 2 : batch) * b.nthr;
+}
+size_t fused_lds_bytes(const Bucket &b, int sets, int batch) { return fused_lds_bytes(b, sets, batch, b.n1); }
+
+// LDS bytes of the two-pass update kernel (8 row sets) with `rows` rows per
+// tile and split n1: the largest of the two-pass and observable kernels.
+size_t update_lds_bytes(const Bucket &b, int rows, int n1) {
+  return size_t(b.esz) * 2 * b.X + size_t(b.esz / 2) * (b.X + (b.X & 1)) +
+         size_t(b.esz) * 8 * rows * row_pad(b.X, b.X / n1) + sizeof(double) * 64;
 }
 
 // rows per workgroup: the largest divisor of R with X*rows <= max_sites
@@ -1059,8 +1112,8 @@ ChainParams to_device_params(const clv_params &p, int eval_every, int active, in
 // twiddles in the order RowFFT reads them ([k1*N2 + n2] = exp(-2 pi i n2 k1 / L),
 // [L + m2*N1 + k1] = exp(+2 pi i m2 k1 / L)) in the field precision, and
 // -p(k)^2 / L with p of compute_kinetic_part (Propagator_without_cutoff.cu:169-176).
-void row_tables_host(int L, int precision, std::vector<char> *tw_bytes, std::vector<double> *p2) {
-  const int n2s = split_n2(L), n1s = L / n2s;
+void row_tables_host(int L, int n1, int precision, std::vector<char> *tw_bytes, std::vector<double> *p2) {
+  const int n1s = n1, n2s = L / n1;
   std::vector<double> tw(4 * L);
   for (int k1 = 0; k1 < n1s; ++k1)
     for (int n2 = 0; n2 < n2s; ++n2) {
@@ -1086,10 +1139,10 @@ void row_tables_host(int L, int precision, std::vector<char> *tw_bytes, std::vec
   }
 }
 
-int upload_row_tables(clv_ctx *ctx, int L, void *d_tw, double *d_p2) {
+int upload_row_tables(clv_ctx *ctx, int L, int n1, void *d_tw, double *d_p2) {
   std::vector<char> tw;
   std::vector<double> p2;
-  row_tables_host(L, ctx->precision, &tw, &p2);
+  row_tables_host(L, n1, ctx->precision, &tw, &p2);
   HIPCHK(ctx, hipMemcpy(d_p2, p2.data(), sizeof(double) * L, hipMemcpyHostToDevice));
   HIPCHK(ctx, hipMemcpy(d_tw, tw.data(), tw.size(), hipMemcpyHostToDevice));
   return CLV_OK;
@@ -1112,7 +1165,7 @@ int enable_kt(clv_ctx *ctx, Bucket &b) {
   for (Bucket::KtAxis &a : axes) {
     if ((rc = dalloc(ctx, b, &a.tw, 2 * b.esz * a.L))) return rc;
     if ((rc = dalloc(ctx, b, reinterpret_cast<void **>(&a.p2), sizeof(double) * a.L))) return rc;
-    if ((rc = upload_row_tables(ctx, a.L, a.tw, a.p2))) return rc;
+    if ((rc = upload_row_tables(ctx, a.L, default_n1(a.L), a.tw, a.p2))) return rc;
     a.W = 1;  // x-lines per workgroup: the largest divisor of X within the budget
     for (int w = 1; w <= b.X; ++w)
       if (b.X % w == 0 && 4 * w * a.L <= budget) a.W = w;
@@ -1134,13 +1187,11 @@ void plan_bucket(Bucket &b, int precision) {
   b.Vs = int64_t(sh.nx) * sh.ny * sh.nz;
   b.S = b.Vs * sh.nt;
   b.esz = (precision == CLV_FP32) ? 8 : 16;
+  b.n1 = default_n1(b.X);
   b.rows = pick_rows(b.X, b.R, std::max(kTileSites, b.X));
-  // the update kernel stages 8 lines per row: keep its LDS within 64 KB
-  const auto update_lds = [&](int rows) {
-    return size_t(b.esz) * 2 * b.X + size_t(b.esz / 2) * (b.X + (b.X & 1)) +
-           size_t(b.esz) * 8 * rows * row_pad(b.X, split_n2(b.X)) + sizeof(double) * 64;
-  };
-  while (b.rows > 1 && update_lds(b.rows) > 65536) {
+  // the update kernel stages 8 lines per row: keep its LDS within 64 KB (the
+  // row tile follows the default split only, so no split changes the tiling)
+  while (b.rows > 1 && update_lds_bytes(b, b.rows, b.n1) > 65536) {
     int q = b.rows - 1;
     while (b.R % q != 0) --q;
     b.rows = q;
@@ -1170,6 +1221,37 @@ void plan_bucket(Bucket &b, int precision) {
   b.schedule = auto_schedule(b, b.nch);
 }
 
+// Switch bucket b to row split n1 (0: the default): new twiddle tables, the
+// fused kernel's LDS options recomputed (fewer line sets or a smaller K_0
+// batch where the new line padding needs it), the fused schedule dropped
+// where it no longer fits.  Results change at the rounding level only (the
+// transform's arithmetic is regrouped), as a different cuFFT plan would.
+int set_row_split(clv_ctx *ctx, Bucket &b, int n1) {
+  if (n1 == 0) n1 = default_n1(b.X);
+  bool known = false;
+  for (int o : split_options(b.X)) known |= (o == n1);
+  if (!known)
+    return fail(ctx, CLV_ERR_ARG, "row split N1=" + std::to_string(n1) + " is not available for X=" +
+                                      std::to_string(b.X));
+  if (update_lds_bytes(b, b.rows, n1) > 65536)
+    return fail(ctx, CLV_ERR_ARG, "row split N1=" + std::to_string(n1) + " does not fit LDS for shape " +
+                                      std::to_string(b.X) + "x" + std::to_string(b.R) + "x" + std::to_string(b.Nt));
+  if (n1 == b.n1) return CLV_OK;
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));  // the tables may be in use
+  int rc = upload_row_tables(ctx, b.X, n1, b.d_tw, b.d_p2);
+  if (rc) return rc;
+  b.n1 = n1;
+  b.fused_sets_max = (fused_lds_bytes(b, 2, 1) <= 65536) ? 2 : 1;
+  if (b.fused_sets > b.fused_sets_max) b.fused_sets = b.fused_sets_max;
+  while (b.fused_batch > 1 && fused_lds_bytes(b, b.fused_sets, b.fused_batch) > 65536) b.fused_batch /= 2;
+  b.fused_lds = fused_lds_bytes(b, b.fused_sets, b.fused_batch);
+  if (b.fused_lds > 65536) b.fused_lds = 0;
+  if (b.schedule == 2 && b.fused_lds == 0) b.schedule = 1;
+  b.primed = false;
+  drop_graphs(ctx);
+  return CLV_OK;
+}
+
 int setup_bucket(clv_ctx *ctx, Bucket &b) {
   plan_bucket(b, ctx->precision);
   const size_t field = size_t(b.nch) * size_t(b.S) * 4 * b.esz;
@@ -1193,7 +1275,7 @@ int setup_bucket(clv_ctx *ctx, Bucket &b) {
   if ((rc = dalloc(ctx, b, reinterpret_cast<void **>(&b.d_F), sizeof(double) * 3 * b.nch * b.Vs))) return rc;
   if ((rc = dalloc(ctx, b, reinterpret_cast<void **>(&b.d_acc), sizeof(double) * 5 * b.nch))) return rc;
   if ((rc = dalloc(ctx, b, reinterpret_cast<void **>(&b.d_count), sizeof(int64_t) * b.nch))) return rc;
-  if ((rc = upload_row_tables(ctx, b.X, b.d_tw, b.d_p2))) return rc;
+  if ((rc = upload_row_tables(ctx, b.X, b.n1, b.d_tw, b.d_p2))) return rc;
   // Zeroed on the context's own stream and waited for: a null-stream memset
   // is not ordered with the context's non-blocking stream, so a field
   // initialisation launched right after could run first and be overwritten.
@@ -1919,16 +2001,16 @@ std::string tune_key(const std::string &dev, const Bucket &b, int precision) {
   return k.str();
 }
 
-// One tuning choice.  All of them give identical results: the schedules are
-// bit-identical, the reduction tile (tcs tau slices per drift-statistics
-// workgroup) does not change the per-(tau, tile) partials, the fused
-// kernel's LDS line sets only change its barriers, and its K_0 batch only
-// groups the same per-line transforms differently.  The row tile and the
-// radix split are fixed per shape: they set the partial-sum granularity and
-// the transform's rounding, so tuning them would make results depend on a
-// timing measurement.
+// One tuning choice.  Within one row split all of them give identical
+// results: the schedules are bit-identical, the reduction tile (tcs tau slices
+// per drift-statistics workgroup) does not change the per-(tau, tile)
+// partials, the fused kernel's LDS line sets only change its barriers, and
+// its K_0 batch only groups the same per-line transforms differently.  The
+// row split n1 regroups the transform's arithmetic (rounding-level changes,
+// as cufftPlanMany's own plan choice, Lattice.cu:135-156).  The row tile is
+// fixed per shape: it sets the partial-sum granularity.
 struct TuneChoice {
-  int schedule = 1, tcs = 1, fsets = 1, fbatch = 1;
+  int schedule = 1, tcs = 1, fsets = 1, fbatch = 1, n1 = 0;
 };
 
 int json_int(const std::string &line, const char *field, int dflt) {
@@ -1937,8 +2019,8 @@ int json_int(const std::string &line, const char *field, int dflt) {
   return p == std::string::npos ? dflt : std::atoi(line.c_str() + p + k.size());
 }
 
-// cache lines: {"key": "...", "schedule": S, "tcs": T, "fsets": F, "fbatch": B, "ms": best, ...,
-//               "candidates": [[schedule, tcs, fsets, fbatch, ms], ...]}
+// cache lines: {"key": "...", "schedule": S, "tcs": T, "fsets": F, "fbatch": B, "n1": N, "ms": best,
+//               ..., "candidates": [[schedule, tcs, fsets, fbatch, n1, ms], ...]}
 std::map<std::string, TuneChoice> read_tune_cache(const char *path) {
   std::map<std::string, TuneChoice> out;
   if (!path) return out;
@@ -1953,6 +2035,7 @@ std::map<std::string, TuneChoice> read_tune_cache(const char *path) {
     c.tcs = json_int(line, "tcs", 0);
     c.fsets = json_int(line, "fsets", 0);
     c.fbatch = json_int(line, "fbatch", 1);
+    c.n1 = json_int(line, "n1", 0);
     out[line.substr(k0 + 8, k1 - k0 - 8)] = c;
   }
   return out;
@@ -2009,30 +2092,39 @@ int restore_snapshot(clv_ctx *ctx, Bucket &b, Snapshot &sn, bool release) {
   return CLV_OK;
 }
 
-// Valid choices for bucket b (reduction tiles that divide Nt and fit LDS;
-// fused line sets and K_0 batches that fit 64 KB, batches of at most Nt).
+// Valid choices for bucket b: every instantiated row split whose kernels fit
+// 64 KB of LDS at the bucket's row tile, and for each the reduction tiles that
+// divide Nt and fit LDS, and the fused line sets and K_0 batches (of at most
+// Nt) that fit 64 KB.
 std::vector<TuneChoice> tune_choices(const Bucket &b) {
   std::vector<TuneChoice> out;
-  const size_t line = size_t(b.esz) * row_pad(b.X, split_n2(b.X));
-  for (int q = 1; q <= 4; ++q)
-    if (b.Nt % q == 0 && q * b.rows * b.X <= 1024 && line * q * b.rows <= 48 * 1024)
-      out.push_back({1, q, 1, 1});
-  if (b.fused_lds > 0 && !b.kt_on)
-    for (int f = 1; f <= b.fused_sets_max; ++f)
-      for (int B = 1; B <= 4; B *= 2)
-        if ((B == 1 || B <= b.Nt) && fused_lds_bytes(b, f, B) <= 65536) out.push_back({2, b.tcs, f, B});
+  for (int n1 : split_options(b.X)) {
+    if (update_lds_bytes(b, b.rows, n1) > 65536) continue;
+    const size_t line = size_t(b.esz) * row_pad(b.X, b.X / n1);
+    for (int q = 1; q <= 4; ++q)
+      if (b.Nt % q == 0 && q * b.rows * b.X <= 1024 && line * q * b.rows <= 48 * 1024)
+        out.push_back({1, q, 1, 1, n1});
+    if (!b.kt_on)
+      for (int f = 1; f <= 2; ++f)
+        for (int B = 1; B <= 4; B *= 2)
+          if ((B == 1 || B <= b.Nt) && fused_lds_bytes(b, f, B, n1) <= 65536) out.push_back({2, b.tcs, f, B, n1});
+  }
   return out;
 }
 
 bool choice_valid(const Bucket &b, const TuneChoice &c) {
   for (const TuneChoice &v : tune_choices(b))
-    if (v.schedule == c.schedule && (c.schedule == 2 || v.tcs == c.tcs) &&
+    if (v.n1 == c.n1 && v.schedule == c.schedule && (c.schedule == 2 || v.tcs == c.tcs) &&
         (c.schedule == 1 || (v.fsets == c.fsets && v.fbatch == c.fbatch)))
       return true;
   return false;
 }
 
-void apply_choice(Bucket &b, const TuneChoice &c) {
+int apply_choice(clv_ctx *ctx, Bucket &b, const TuneChoice &c) {
+  if (c.n1 != b.n1) {
+    int rc = set_row_split(ctx, b, c.n1);
+    if (rc) return rc;
+  }
   b.schedule = c.schedule;
   if (c.schedule == 1) {
     b.tcs = c.tcs;
@@ -2042,11 +2134,12 @@ void apply_choice(Bucket &b, const TuneChoice &c) {
     b.fused_lds = fused_lds_bytes(b, c.fsets, c.fbatch);
   }
   b.primed = false;
+  return CLV_OK;
 }
 
 // ms per step of bucket b under choice c (eager steps, no observables)
 int time_choice(clv_ctx *ctx, Bucket &b, const TuneChoice &c, int nsteps, double *ms) {
-  apply_choice(b, c);
+  if (int rc = apply_choice(ctx, b, c)) return rc;
   for (int n = 0; n < 2; ++n) step_bucket(ctx, b, 0);
   hipEvent_t e0, e1;
   HIPCHK(ctx, hipEventCreate(&e0));
@@ -2080,12 +2173,12 @@ int clv_autotune(clv_ctx *ctx, const char *cache_path) {
     const std::string key = tune_key(dev, b, ctx->precision) + "|" + std::to_string(b.nch) + "ch";
     auto hit = cache.find(key);
     if (hit != cache.end() && choice_valid(b, hit->second)) {
-      apply_choice(b, hit->second);
+      if ((rc = apply_choice(ctx, b, hit->second))) break;
       continue;
     }
     const std::vector<TuneChoice> cand = tune_choices(b);
     if (cand.size() <= 1) {
-      if (!cand.empty()) apply_choice(b, cand[0]);
+      if (!cand.empty() && (rc = apply_choice(ctx, b, cand[0]))) break;
       continue;
     }
     Snapshot sn;
@@ -2104,7 +2197,7 @@ int clv_autotune(clv_ctx *ctx, const char *cache_path) {
     size_t best = 0;
     for (size_t q = 1; q < cand.size(); ++q)
       if (ms[q] < ms[best]) best = q;
-    apply_choice(b, cand[best]);
+    if ((rc = apply_choice(ctx, b, cand[best]))) break;
     if (cache_path) {
       double two = 0.0, fused = 0.0;  // best of each schedule
       for (size_t q = 0; q < cand.size(); ++q) {
@@ -2114,11 +2207,12 @@ int clv_autotune(clv_ctx *ctx, const char *cache_path) {
       std::ofstream out(cache_path, std::ios::app);
       out << "{\"key\": \"" << key << "\", \"schedule\": " << cand[best].schedule
           << ", \"tcs\": " << b.tcs << ", \"fsets\": " << b.fused_sets << ", \"fbatch\": " << b.fused_batch
+          << ", \"n1\": " << b.n1
           << ", \"ms\": " << ms[best]
           << ", \"ms_two_pass\": " << two << ", \"ms_fused\": " << fused << ", \"candidates\": [";
       for (size_t q = 0; q < cand.size(); ++q)
         out << (q ? ", " : "") << "[" << cand[q].schedule << ", " << cand[q].tcs << ", " << cand[q].fsets
-            << ", " << cand[q].fbatch << ", " << ms[q] << "]";
+            << ", " << cand[q].fbatch << ", " << cand[q].n1 << ", " << ms[q] << "]";
       out << "]}\n";
     }
   }
@@ -2194,6 +2288,19 @@ int clv_kernel_time(clv_ctx *ctx, int32_t kind, double *total_ms, int64_t *launc
   if (total_ms) *total_ms = ctx->kt_ms[kind];
   if (launches) *launches = ctx->kt_n[kind];
   return CLV_OK;
+}
+
+int clv_set_row_split(clv_ctx *ctx, int32_t chain, int32_t n1) {
+  int rc = check_chain(ctx, chain);
+  if (rc) return rc;
+  if (n1 < 0) return fail(ctx, CLV_ERR_ARG, "clv_set_row_split: n1 < 0");
+  (void)hipSetDevice(ctx->device);
+  return set_row_split(ctx, ctx->buckets[ctx->where[chain].first], n1);
+}
+
+int clv_get_row_split(const clv_ctx *ctx, int32_t chain) {
+  if (!ctx || chain < 0 || chain >= int(ctx->where.size())) return CLV_ERR_ARG;
+  return ctx->buckets[ctx->where[chain].first].n1;
 }
 
 int clv_get_schedule(const clv_ctx *ctx, int32_t chain) {
@@ -2342,7 +2449,7 @@ extern "C" int clv_step_noalloc(const clv_shape *shape, int32_t precision, const
   st.parity = 0;  // the inputs are buffer 0, the outputs buffer 1
   std::vector<char> tw;
   std::vector<double> p2;
-  row_tables_host(b.X, precision, &tw, &p2);
+  row_tables_host(b.X, b.n1, precision, &tw, &p2);
   clv_ctx ctx;  // launch record only: no device memory, the caller's stream
   ctx.device = device;
   ctx.precision = precision;

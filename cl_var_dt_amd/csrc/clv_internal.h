@@ -27,6 +27,7 @@ struct Bucket {
   int nch = 0;
   std::vector<int> ids;  // global chain ids (context buckets)
   int rows = 1, ntiles = 1, nthr = 64, tc = 1, nchunks = 1, tcs = 1;
+  int n1 = 1;            // row transform split X = n1 * (X / n1)
   size_t esz = 0;        // bytes per complex element
   void *psi[2] = {nullptr, nullptr}, *psib[2] = {nullptr, nullptr};
   ChainParams *d_prm = nullptr;
@@ -122,6 +123,8 @@ bool supported_x(int X);
 // per-chain device bytes of a bucket of this shape and precision
 int64_t chain_bytes(const clv_shape &s, int precision);
 int setup_bucket(clv_ctx *ctx, Bucket &b);
+// Row-transform split of a bucket (0: the default for its row length).
+int set_row_split(clv_ctx *ctx, Bucket &b, int n1);
 void free_bucket(clv_ctx *ctx, Bucket &b);
 ChainParams to_device_params(const clv_params &p, int eval_every, int active,
                              int64_t eval_from = 0);

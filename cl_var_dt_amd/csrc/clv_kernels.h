@@ -295,9 +295,9 @@ __device__ __forceinline__ double drift_abs(C2<T> d) {
 
 // Shared-memory carve-up of the row-transform kernels: twiddles, the
 // kinetic multiplier table, `nlines` padded lines, reduction scratch.
-template <typename T, int X>
+template <typename T, int X, int N1 = Split<X>::N1>
 struct Smem {
-  using F = RowFFT<T, X>;
+  using F = RowFFT<T, X, N1>;
   C2<T> *tw;
   T *mult;
   C2<T> *buf;
@@ -321,8 +321,8 @@ struct Smem {
 // Twiddles and the kinetic multiplier m(k) = scale * (-p_x(k)^2 / X) into LDS:
 // scale = 1/a_x gives K (compute_kinetic_part + normalize, Propagator_without_cutoff.cu:160-203,
 // 423-424); the drift kernels use drift_table_scale.
-template <typename T, int X>
-__device__ __forceinline__ void load_tables(Smem<T, X> &sm, const C2<T> *tw, const double *p2,
+template <typename T, int X, int N1>
+__device__ __forceinline__ void load_tables(Smem<T, X, N1> &sm, const C2<T> *tw, const double *p2,
                                             double scale, int tid, int nthr) {
   for (int e = tid; e < X; e += nthr) {
     sm.tw[e] = tw[e];

@@ -142,6 +142,13 @@ class Engine:
     def schedule(self, chain: int = 0) -> int:
         return int(lib().clv_get_schedule(self._h, chain))
 
+    def set_row_split(self, chain: int, n1: int) -> None:
+        """Row-transform split N1 (X = N1 * N2) of the chain's shape; 0 = default."""
+        self._chk(lib().clv_set_row_split(self._h, chain, n1))
+
+    def row_split(self, chain: int = 0) -> int:
+        return int(lib().clv_get_row_split(self._h, chain))
+
     def set_fused_options(self, line_sets: int = 0, batch: int = 0) -> None:
         """Fused schedule: LDS line sets (1, 2) and K_0 batch (1, 2, 4); 0 keeps."""
         self._chk(lib().clv_set_fused_options(self._h, line_sets, batch))

@@ -178,8 +178,9 @@ int clv_read_observables(clv_ctx *ctx, int32_t chain, clv_observables *out,
 int clv_reset_observables(clv_ctx *ctx, int32_t chain);
 
 /* ---- kernel selection (project autotuner) ------------------------------ */
-/* Times the candidate kernel schedules of every shape bucket of ctx on its
- * current state (state is saved and restored) and keeps the fastest.  When
+/* Times the candidate kernel schedules and row-transform splits of every
+ * shape bucket of ctx on its current state (state is saved and restored) and
+ * keeps the fastest.  When
  * cache_path is non-NULL the choice is read from / appended to that JSON-lines
  * file, keyed by shape, precision and device name. */
 int clv_autotune(clv_ctx *ctx, const char *cache_path);
@@ -189,6 +190,17 @@ int clv_autotune(clv_ctx *ctx, const char *cache_path);
 int clv_set_schedule(clv_ctx *ctx, int32_t schedule);
 /* The schedule the chain's bucket runs (1 or 2). */
 int clv_get_schedule(const clv_ctx *ctx, int32_t chain);
+/* Radix split of the row transform of the chain's bucket (every chain of its
+ * shape): the length N1 of the first level of the two-level transform of
+ * rows of X sites, X = N1 * N2 -- e.g. 128 = 16 x 8 (default) or 8 x 16.  The
+ * reference leaves this choice to cufftPlanMany (Lattice.cu:135-156).  Splits
+ * regroup the transform's arithmetic, so results differ at the rounding level
+ * only (FP64 ~1e-16, FP32 ~1e-7 relative per step); clv_autotune times them.
+ * n1 = 0 restores the default; CLV_ERR_ARG for a split this library does not
+ * build for X or that does not fit LDS at the shape's row tile.
+ * clv_get_row_split returns the bucket's N1. */
+int clv_set_row_split(clv_ctx *ctx, int32_t chain, int32_t n1);
+int clv_get_row_split(const clv_ctx *ctx, int32_t chain);
 /* Options of the fused schedule for every bucket that has it (project
  * extension, no counterpart in the reference).  line_sets: LDS line sets,
  * 1 or 2 (two alternate by tau parity); batch: tau slices whose next-step

@@ -178,7 +178,9 @@ def test_cancel_frees_the_slot_and_leaves_other_runs_unchanged():
     n_keep = 20000
     with serving.Server(precision=CLV_FP32) as srv:
         alone = srv.wait(srv.submit(a, pa, nsteps=n_keep, eval_every=100), timeout=300, release=True)
-    budget = int(2.5 * 4 * a.sites * 4 * 8)   # one pool of two a-slots, no room for a b-pool
+    # ~20 MB per a-slot and ~40 MB per b-slot (fields, accumulators): one pool
+    # of two a-slots (35 MB), no room for a b-pool beside it
+    budget = 50 << 20
     with serving.Server(precision=CLV_FP32, hbm_budget_bytes=budget) as srv:
         long_run = srv.submit(a, _p(52, nt=32), nsteps=10 ** 7)
         keep = srv.submit(a, pa, nsteps=n_keep, eval_every=100)
