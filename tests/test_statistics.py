@@ -8,9 +8,17 @@
    standard error; the engine must agree within 2 sigma (plus the O(dt) bias
    of Euler-Maruyama, bounded here by a 1% allowance).  This pins the kinetic
    sign and normalisation, the drift, the noise variance and the observable.
-2. FP32 batched chains against the FP64 CPU oracle on the same Philox
-   streams: density, Sz and energy means over independent chains agree within
-   two standard errors; the measured differences are printed.
+2. FP32 batched chains against the FP64 CPU oracle, and FP32 engine chains
+   against FP64 engine chains, on the same Philox streams (chain i has seed i
+   in both).  Two checks per observable, each printed:
+   - statistical: the two means agree within two standard errors of the
+     independent-chain spread (the north-star criterion);
+   - paired: the per-chain differences d_i = fp32_i - fp64_i have a mean
+     within two standard errors of the d_i themselves plus the FP32 rounding
+     allowance 1e-5 (|mean| + chain spread) -- a bias ten times the observed
+     one fails this,
+     where the statistical check alone would pass biases a thousand times
+     larger.
 """
 import numpy as np
 import pytest
@@ -23,6 +31,25 @@ pytestmark = pytest.mark.gpu
 def _engine(shapes, precision):
     from cl_var_dt_amd.engine import Engine
     return Engine(shapes, precision=precision)
+
+
+NAMES = ["density", "Sz", "energy"]
+
+
+def _compare(a, b, label, rel=1e-5):
+    """a: FP32 runs, b: FP64 runs of the same seeds, shape (chains, observables)."""
+    n = a.shape[0]
+    for k, name in enumerate(NAMES):
+        x, y = a[:, k], b[:, k]
+        se = np.sqrt(x.var(ddof=1) / n + y.var(ddof=1) / n)
+        d = x - y
+        se_d = d.std(ddof=1) / np.sqrt(n)
+        allow = rel * (abs(y.mean()) + y.std(ddof=1))
+        print(f"{label} {name}: fp32 {x.mean():.8f} fp64 {y.mean():.8f} diff {d.mean():+.2e} | "
+              f"statistical 2 sigma {2 * se:.2e} | paired mean {d.mean():+.2e} 2 sigma_d {2 * se_d:.2e} "
+              f"+ allowance {allow:.2e} | max |d_i| {np.max(abs(d)):.2e}")
+        assert abs(x.mean() - y.mean()) <= 2 * se + 1e-12, name
+        assert abs(d.mean()) <= 2 * se_d + allow, name
 
 
 def test_free_theory_occupation():
@@ -73,19 +100,45 @@ def test_fp32_observables_match_fp64_oracle(oracle_lib):
         ob, _ = orc.observables()
         o.append([ob.density, ob.Sz, ob.energy])
         orc.close()
-    o = np.array(o)
-    for k, name in enumerate(["density", "Sz", "energy"]):
-        se = np.sqrt(g[:, k].var(ddof=1) / nch + o[:, k].var(ddof=1) / nch)
-        diff = g[:, k].mean() - o[:, k].mean()
-        print(f"{name}: fp32 {g[:, k].mean():.6f}  fp64 oracle {o[:, k].mean():.6f}  "
-              f"diff {diff:+.2e}  2 sigma {2 * se:.2e}")
-        assert abs(diff) <= 2 * se + 1e-12
+    _compare(g, np.array(o), "8x8 vs oracle")
+
+
+def test_fp32_observables_match_fp64_oracle_2d(oracle_lib):
+    """A 2-D lattice, 32^2 x 16, 16 independent chains (seeds 4100 + i) of the
+    FP32 engine against the FP64 oracle on the same seeds: 300 steps to relax,
+    then 1200 with observables every 10."""
+    print("case: fp32 vs oracle observables 32x32x16", flush=True)
+    import os
+    shape = Shape(32, 32, 1, 16)
+    nch, therm, steps = 16, 300, 1200
+    kw = dict(mu=1.0, G=0.1, c1=-0.05, Q=0.1, eps=1 / 16, DT=5e-3)
+    eng = _engine([shape] * nch, CLV_FP32)
+    for i in range(nch):
+        eng.set_params(i, default_params(seed=4100 + i, **kw))
+        eng.init_fields(i, 1.0, 0.1)
+    eng.propagate(therm, 0)
+    eng.reset_observables()
+    eng.propagate(steps, 10)
+    g = np.array([[getattr(eng.observables(i)[0], f) for f in NAMES] for i in range(nch)])
+    eng.close()
+    threads = max(1, min(16, len(os.sched_getaffinity(0))))
+    o = []
+    for i in range(nch):
+        orc = oracle_lib.Oracle(shape, default_params(seed=4100 + i, **kw), nthreads=threads)
+        orc.init_fields(1.0, 0.1)
+        orc.propagate(therm, 0)
+        orc.reset_observables()
+        orc.propagate(steps, 10)
+        ob, _ = orc.observables()
+        o.append([getattr(ob, f) for f in NAMES])
+        orc.close()
+    _compare(g, np.array(o), "32x32x16 vs oracle")
 
 
 def test_fp32_and_fp64_engine_chains_agree():
     """The 2-D configuration's couplings (mu = 1, c0 = 0.1, c1 = -0.05, q = 0.1)
-    on a reduced 16^2 x 16 lattice: FP32 and FP64 chains, density / Sz /
-    energy within two standard errors."""
+    on a reduced 16^2 x 16 lattice: FP32 and FP64 chains of the same seeds,
+    density / Sz / energy by the statistical and the paired check."""
     print("case: fp32 vs fp64 engine", flush=True)
     shape = Shape(16, 16, 1, 16)
     nch = 24
@@ -102,12 +155,7 @@ def test_fp32_and_fp64_engine_chains_agree():
         res[prec] = np.array([[eng.observables(i)[0].density, eng.observables(i)[0].Sz,
                                eng.observables(i)[0].energy] for i in range(nch)])
         eng.close()
-    for k, name in enumerate(["density", "Sz", "energy"]):
-        a, b = res[CLV_FP32][:, k], res[CLV_FP64][:, k]
-        se = np.sqrt(a.var(ddof=1) / nch + b.var(ddof=1) / nch)
-        print(f"{name}: fp32 {a.mean():.6f} fp64 {b.mean():.6f} diff {a.mean() - b.mean():+.2e} "
-              f"2 sigma {2 * se:.2e} max per-chain |diff| {np.max(abs(a - b)):.2e}")
-        assert abs(a.mean() - b.mean()) <= 2 * se + 1e-12
+    _compare(res[CLV_FP32], res[CLV_FP64], "16x16x16")
 
 
 @pytest.mark.parametrize("c1", [-0.05, 0.05])
@@ -116,8 +164,8 @@ def test_config1_fp32_vs_fp64_observables(c1):
     mu = 1, c0 = 0.1, q = 0.1, c1 in {-0.05, +0.05}, 5x10^4 adaptive steps,
     Gaussian start around sqrt(n0 = 1), observables every 10 steps accumulated
     in FP64 from step 0 (as the reference does).  16 independent chains per
-    precision (seeds 7, 8, ...): the FP32 and FP64 means of density, Sz and
-    energy agree within two standard errors of their difference."""
+    precision (seeds 7, 8, ...): density, Sz and energy by the statistical and
+    the paired check."""
     print("case: config 64x64x64 c1 =", c1, flush=True)
     shape = Shape(64, 64, 1, 64)
     nch, steps = 16, 50000
@@ -133,10 +181,4 @@ def test_config1_fp32_vs_fp64_observables(c1):
                                eng.observables(i)[0].energy] for i in range(nch)])
         assert all(eng.stats(i).steps == steps and eng.stats(i).nonfinite == 0 for i in range(nch))
         eng.close()
-    for k, name in enumerate(["density", "Sz", "energy"]):
-        a, b = res[CLV_FP32][:, k], res[CLV_FP64][:, k]
-        se = np.sqrt(a.var(ddof=1) / nch + b.var(ddof=1) / nch)
-        print(f"c1={c1:+.2f} {name}: fp32 {a.mean():.6f} fp64 {b.mean():.6f} "
-              f"diff {a.mean() - b.mean():+.2e} 2 sigma {2 * se:.2e} "
-              f"max per-chain |diff| {np.max(abs(a - b)):.2e}")
-        assert abs(a.mean() - b.mean()) <= 2 * se + 1e-12
+    _compare(res[CLV_FP32], res[CLV_FP64], f"64x64x64 c1={c1:+.2f}")
