@@ -60,7 +60,7 @@ def measured_valu(kernel: str, x: int, lattice: str, chains: int):
         return None, None
     for line in open(path):
         cells = [c.strip() for c in line.strip().strip("|").split("|")]
-        if len(cells) >= 4 and cells[0] == f"`k_{kernel}<float, {x}>`" and cells[3] not in ("", "-"):
+        if len(cells) >= 4 and cells[0].startswith(f"`k_{kernel}<float, {x},") and cells[3] not in ("", "-"):
             return float(cells[3]), os.path.relpath(path, ROOT)
     return None, None
 
