@@ -59,15 +59,21 @@ struct ChainState {
 };
 
 // ----------------------------------------------------------------- Philox --
+// a ^ b ^ c in one instruction: gfx950's three-input bitwise op with the
+// truth table of XOR3 (0x96).
+__device__ __forceinline__ uint32_t xor3(uint32_t a, uint32_t b, uint32_t c) {
+  return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96);
+}
+
 // Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11).  Each round uses two
-// 32x32->64 products (v_mad_u64_u32).
+// 32x32->64 products (v_mad_u64_u32) and two three-input XORs.
 __device__ __forceinline__ uint4 philox10(uint4 c, uint32_t k0, uint32_t k1) {
 #pragma unroll
   for (int r = 0; r < 10; ++r) {
     const uint64_t p0 = uint64_t(0xD2511F53u) * c.x;
     const uint64_t p1 = uint64_t(0xCD9E8D57u) * c.z;
-    const uint32_t n0 = uint32_t(p1 >> 32) ^ c.y ^ k0;
-    const uint32_t n2 = uint32_t(p0 >> 32) ^ c.w ^ k1;
+    const uint32_t n0 = xor3(uint32_t(p1 >> 32), c.y, k0);
+    const uint32_t n2 = xor3(uint32_t(p0 >> 32), c.w, k1);
     c = make_uint4(n0, uint32_t(p1), n2, uint32_t(p0));
     k0 += 0x9E3779B9u;
     k1 += 0xBB67AE85u;
