@@ -31,6 +31,13 @@ __device__ __forceinline__ C2<T> cconj(C2<T> a) { return {a.x, -a.y}; }
 template <int S, typename T>
 __device__ __forceinline__ C2<T> cmul_si(C2<T> a) { return {T(-S) * a.y, T(S) * a.x}; }
 
+// a * b + c with one rounding, in the field precision
+template <typename T>
+__device__ __forceinline__ T fmad(T a, T b, T c) {
+  if constexpr (sizeof(T) == 4) return __builtin_fmaf(a, b, c);
+  else return __builtin_fma(a, b, c);
+}
+
 // ------------------------------------------------------- per-chain records --
 // Per-chain parameters as the kernels read them (clv_params, converted).
 struct ChainParams {
@@ -198,10 +205,94 @@ __device__ __forceinline__ void dft4(C2<T> *x) {
 template <int R, int S, typename T>
 __device__ __forceinline__ void dft(C2<T> *x);
 
+// a + g b and a + g (S i) b for a constant g: an add where g = +-1, else
+// one FMA per real part.
+template <typename T>
+__device__ __forceinline__ C2<T> caxpy(C2<T> a, double g, C2<T> b) {
+  if (g == 1.0) return cadd(a, b);
+  if (g == -1.0) return csub(a, b);
+  return {fmad(T(g), b.x, a.x), fmad(T(g), b.y, a.y)};
+}
+template <int S, typename T>
+__device__ __forceinline__ C2<T> caxpy_i(C2<T> a, double g, C2<T> b) {
+  const double h = S * g;
+  if (h == 1.0) return {a.x - b.y, a.y + b.x};
+  if (h == -1.0) return {a.x + b.y, a.y - b.x};
+  return {fmad(T(-h), b.y, a.x), fmad(T(h), b.x, a.y)};
+}
+
+// x exp(S 2 pi i m / R) = g (x u) with x u cheap: quarter turns (g = 1, u one
+// of 1, -1, i, -i: no arithmetic), odd eighth turns (g = 1/sqrt 2,
+// u = +-1 +- i: two adds), other angles (u = 1 + i tan with g = cos, or
+// cot + i with g = sin, whichever ratio is at most 1: two FMAs).  Returns x u
+// and sets *g; the caller folds g into the FMAs of the next butterflies
+// (Linzer and Feig 1993, "Modified FFTs for fused multiply-add
+// architectures"), so an inner twiddle costs 0 or 2 instructions, not 4.
+template <int S, typename T>
+__device__ __forceinline__ C2<T> twiddle_unit(C2<T> x, int m, int R, double *g) {
+  const double c = kcos(m, R), s = S * ksin(m, R);
+  *g = 1.0;
+  if (s == 0.0) return c > 0.0 ? x : C2<T>{-x.x, -x.y};
+  if (c == 0.0) return s > 0.0 ? C2<T>{-x.y, x.x} : C2<T>{x.y, -x.x};
+  const double ac = c < 0.0 ? -c : c, as = s < 0.0 ? -s : s;
+  if (ac == as) {
+    *g = ac;
+    const C2<T> xc = c > 0.0 ? x : C2<T>{-x.x, -x.y};  // x sign(c)
+    const C2<T> xs = s > 0.0 ? x : C2<T>{-x.x, -x.y};  // x sign(s)
+    return {xc.x - xs.y, xc.y + xs.x};
+  }
+  if (ac >= as) {
+    *g = c;
+    const T t = T(s / c);
+    return {fmad(-x.y, t, x.x), fmad(x.x, t, x.y)};
+  }
+  *g = s;
+  const T t = T(c / s);
+  return {fmad(x.x, t, -x.y), fmad(x.y, t, x.x)};
+}
+
+// DFT_B of a_n = g_n b_n (g_0 = 1) given b in x: the scales ride on the
+// butterflies' FMAs, so this costs what the unscaled DFT_B costs.
+template <int S, typename T>
+__device__ __forceinline__ void dft2_scaled(C2<T> *x, const double *g) {
+  const C2<T> a = x[0], b = x[1];
+  x[0] = caxpy(a, g[1], b);
+  x[1] = caxpy(a, -g[1], b);
+}
+template <int S, typename T>
+__device__ __forceinline__ void dft3_scaled(C2<T> *x, const double *g) {
+  const double r = g[2] / g[1], h = 0.86602540378443865;  // sqrt(3)/2
+  const C2<T> t = caxpy(x[1], r, x[2]);   // (a_1 + a_2) / g_1
+  const C2<T> d = caxpy(x[1], -r, x[2]);  // (a_1 - a_2) / g_1
+  const C2<T> m = caxpy(x[0], -0.5 * g[1], t);
+  x[0] = caxpy(x[0], g[1], t);
+  x[1] = caxpy_i<S>(m, h * g[1], d);
+  x[2] = caxpy_i<S>(m, -h * g[1], d);
+}
+template <int S, typename T>
+__device__ __forceinline__ void dft4_scaled(C2<T> *x, const double *g) {
+  const double r = g[3] / g[1];
+  const C2<T> s02 = caxpy(x[0], g[2], x[2]), d02 = caxpy(x[0], -g[2], x[2]);
+  const C2<T> s13 = caxpy(x[1], r, x[3]), d13 = caxpy(x[1], -r, x[3]);  // / g_1
+  x[0] = caxpy(s02, g[1], s13);
+  x[2] = caxpy(s02, -g[1], s13);
+  x[1] = caxpy_i<S>(d02, g[1], d13);
+  x[3] = caxpy_i<S>(d02, -g[1], d13);
+}
+template <int B, int S, typename T>
+__device__ __forceinline__ void dft_scaled(C2<T> *x, const double *g) {
+  if constexpr (B == 2) dft2_scaled<S>(x, g);
+  else if constexpr (B == 3) dft3_scaled<S>(x, g);
+  else dft4_scaled<S>(x, g);
+}
+
 // R = A*B: n = n2 + B n1, k = k1 + A k2 (decimation as in Cooley-Tukey, 1965).
+// The inner twiddles W_R^{n2 k1} enter as twiddle_unit and scaled DFT_B.
 template <int A, int B, int S, typename T>
 __device__ __forceinline__ void dft_split(C2<T> *x) {
+  static_assert(B >= 2 && B <= 4, "second factor of a split DFT");
   C2<T> t[A * B];
+  double g[A * B];
 #pragma unroll
   for (int n2 = 0; n2 < B; ++n2) {
     C2<T> v[A];
@@ -209,22 +300,19 @@ __device__ __forceinline__ void dft_split(C2<T> *x) {
     for (int n1 = 0; n1 < A; ++n1) v[n1] = x[n2 + B * n1];
     dft<A, S>(v);
 #pragma unroll
-    for (int k1 = 0; k1 < A; ++k1) {
-      const int m = n2 * k1;
-      if (m % (A * B) == 0) {
-        t[n2 * A + k1] = v[k1];
-      } else {
-        const C2<T> w = {T(kcos(m, A * B)), T(S * ksin(m, A * B))};
-        t[n2 * A + k1] = cmul(v[k1], w);
-      }
-    }
+    for (int k1 = 0; k1 < A; ++k1)
+      t[n2 * A + k1] = twiddle_unit<S>(v[k1], n2 * k1, A * B, &g[n2 * A + k1]);
   }
 #pragma unroll
   for (int k1 = 0; k1 < A; ++k1) {
     C2<T> v[B];
+    double gk[B];
 #pragma unroll
-    for (int n2 = 0; n2 < B; ++n2) v[n2] = t[n2 * A + k1];
-    dft<B, S>(v);
+    for (int n2 = 0; n2 < B; ++n2) {
+      v[n2] = t[n2 * A + k1];
+      gk[n2] = g[n2 * A + k1];
+    }
+    dft_scaled<B, S>(v, gk);
 #pragma unroll
     for (int k2 = 0; k2 < B; ++k2) x[k1 + A * k2] = v[k2];
   }

@@ -155,13 +155,6 @@ __device__ __forceinline__ C2<T> cdet(const C2<T> a[4]) {
           a[0].x * a[3].y + a[0].y * a[3].x - (a[1].x * a[2].y + a[1].y * a[2].x)};
 }
 
-// a * b + c with one rounding, in the field precision
-template <typename T>
-__device__ __forceinline__ T fmad(T a, T b, T c) {
-  if constexpr (sizeof(T) == 4) return __builtin_fmaf(a, b, c);
-  else return __builtin_fma(a, b, c);
-}
-
 // One component of a drift: af (f1 - f0) + kin + c f1' + B cof, with
 // f1' = f1 except in the psibar equation of the reference conj-drift mode.  Written
 // as one chain of FMAs (the complex products' negations are free operand
