@@ -308,6 +308,10 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? kFusedWavesPerEU : 2) void k_
   const T dtT = T(dt);
   const T sq = T(sqrt(dt));
   auto site_at = [&](int t) { return (int64_t(t) * A.R + r) * X + x; };
+  // In a line set the 8 row-set lines of row j are adjacent (line 8 j + c),
+  // so a thread's cells sit at fixed offsets from one address; the parked
+  // K_0 slots follow them (line 8 rows + q rows + j).
+  C2<T> *const cell = sm.buf + (has_site ? 8 * j : 0) * RF::PAD + RF::pos(x);
   // Registers carried along tau (old = fields of this step, new = updated):
   //   pm = old psi(t-1), km = K(t-1), nprev = new psi(t-1),
   //   p0 = old psi(t), b0 = old psibar(t), bp = old psibar(t+1).
@@ -368,10 +372,11 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? kFusedWavesPerEU : 2) void k_
       load4(psi, site_at(tp1), pp);
       // old psibar(t+2) (tau = 0 at t = Nt-2)
       if (t + 2 <= Nt) load4(psib, site_at(t + 2 == Nt ? 0 : t + 2), bn);
+      C2<T> *const cl = cell + L0 * RF::PAD;
 #pragma unroll
       for (int c = 0; c < 4; ++c) {
-        sm.at(L0 + c * rows + j, x) = p0[c];
-        sm.at(L0 + (4 + c) * rows + j, x) = bp[c];
+        cl[c * RF::PAD] = p0[c];
+        cl[(4 + c) * RF::PAD] = bp[c];
       }
       // K_0 input of tau = t (new psi_0(t-1)); at a batch tau into its own
       // slot of this set, otherwise parked after this tau's first barrier
@@ -394,10 +399,11 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? kFusedWavesPerEU : 2) void k_
     double v_batch = 0.0;
     if (has_site) {
       C2<T> kn[4], kp[4];
+      const C2<T> *const cl = cell + L0 * RF::PAD;
 #pragma unroll
       for (int c = 0; c < 4; ++c) {
-        kn[c] = sm.at(L0 + c * rows + j, x);
-        kp[c] = sm.at(L0 + (4 + c) * rows + j, x);
+        kn[c] = cl[c * RF::PAD];
+        kp[c] = cl[(4 + c) * RF::PAD];
       }
       C2<T> D[4], Db[4];
       drift_site(k, pm, p0, pp, b0, bp, km, kp, D, Db);
@@ -514,7 +520,10 @@ __global__ __launch_bounds__(256) void k_obs(BucketArgs<T> A) {
       load4(psib, site_at(tu + 1 == Nt ? 0 : tu + 1), bd[u]);
     }
   };
-  // a batch's fields are loaded one batch ahead, overlapping the transform
+  // a batch's fields are loaded one batch ahead, overlapping the transform;
+  // slice u's 4 lines of row j are adjacent (line 4 rows u + 4 j + c)
+  constexpr int PAD = RowFFT<T, X, N1>::PAD;
+  C2<T> *const cell = sm.buf + (has_site ? 4 * j : 0) * PAD + RowFFT<T, X, N1>::pos(x);
   C2<T> q[S][4], b[S][4];
   if (has_site) load_pair(chunk * tc, tc < S ? tc : S, q, b);
   for (int tt = 0; tt < tc; tt += S) {
@@ -525,7 +534,7 @@ __global__ __launch_bounds__(256) void k_obs(BucketArgs<T> A) {
       for (int u = 0; u < S; ++u) {
         if (u >= cnt) break;
 #pragma unroll
-        for (int c = 0; c < 4; ++c) sm.at((4 * u + c) * rows + j, x) = q[u][c];
+        for (int c = 0; c < 4; ++c) cell[(4 * rows * u + c) * PAD] = q[u][c];
       }
       if (tt + S < tc) load_pair(t + S, tc - tt - S < S ? tc - tt - S : S, qn, bn);
     }
@@ -538,7 +547,7 @@ __global__ __launch_bounds__(256) void k_obs(BucketArgs<T> A) {
         C2<double> bb[4], qq[4], kk[4];
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
-          const C2<T> kc = sm.at((4 * u + c) * rows + j, x);
+          const C2<T> kc = cell[(4 * rows * u + c) * PAD];
           bb[c] = {double(b[u][c].x), double(b[u][c].y)};
           qq[c] = {double(q[u][c].x), double(q[u][c].y)};
           kk[c] = {double(kc.x), double(kc.y)};
