@@ -319,6 +319,7 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? kFusedWavesPerEU : 2) void k_
   // prefetches old psibar(t+2) (the next iteration's transform input), so
   // every slice is read from memory once and the loads overlap the transform.
   C2<T> pm[4], km[4], nprev[4], p0[4], b0[4], bp[4];
+  C2<T> an = {T(0), T(0)};  // kA N of the pair (pm, b0), carried along tau
 #pragma unroll
   for (int c = 0; c < 4; ++c) nprev[c] = {T(0), T(0)};
   if (has_site) {
@@ -334,6 +335,7 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? kFusedWavesPerEU : 2) void k_
   if (has_site) {
 #pragma unroll
     for (int c = 0; c < 4; ++c) km[c] = sm.at(c * rows + j, x);
+    an = drift_an(k, pm, b0);
   }
   __syncthreads();  // the LDS lines are rewritten below
   // With two LDS sets the transform lines alternate by tau parity, so a tau's
@@ -406,7 +408,7 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? kFusedWavesPerEU : 2) void k_
         kp[c] = cl[(4 + c) * RF::PAD];
       }
       C2<T> D[4], Db[4];
-      drift_site(k, pm, p0, pp, b0, bp, km, kp, D, Db);
+      an = drift_site_carry(k, pm, p0, pp, b0, bp, km, kp, an, D, Db);
       T nz[8] = {};
       if (P.noise_on) {
         const uint32_t site = uint32_t((int64_t(t) * X + x) * A.R + r);

@@ -206,24 +206,36 @@ __device__ __forceinline__ void update_site(T dt, const C2<T> p0[4], const C2<T>
 // psibar equation is the same with the roles of Psi(t) and Psibar(t+1) swapped.
 //   pm = psi(t-1), p0 = psi(t), pp = psi(t+1), b0 = psibar(t), bp = psibar(t+1)
 //   km = K'(t-1), kp = Kb'(t+1) (already scaled by a_x f eps)
+// kA N of a time-split pair (a, b): N = sum_j a_j b_j.
 template <typename T>
-__device__ __forceinline__ void drift_site(const Coef<T> &k, const C2<T> pm[4], const C2<T> p0[4],
-                                           const C2<T> pp[4], const C2<T> b0[4], const C2<T> bp[4],
-                                           const C2<T> km[4], const C2<T> kp[4], C2<T> D[4],
-                                           C2<T> Db[4]) {
+__device__ __forceinline__ C2<T> drift_an(const Coef<T> &k, const C2<T> a[4], const C2<T> b[4]) {
+  const C2<T> N = cdot4(a, b);
+  return {k.kA * N.x, k.kA * N.y};
+}
+
+// drift_site with kA N of the psi pair (pm, b0) given as anm; returns kA N of
+// the psibar pair (p0, bp), which is the next tau-slice's psi pair, so a march
+// along tau forms each N once.
+template <typename T>
+__device__ __forceinline__ C2<T> drift_site_carry(const Coef<T> &k, const C2<T> pm[4],
+                                                  const C2<T> p0[4], const C2<T> pp[4],
+                                                  const C2<T> b0[4], const C2<T> bp[4],
+                                                  const C2<T> km[4], const C2<T> kp[4], C2<T> anm,
+                                                  C2<T> D[4], C2<T> Db[4]) {
   {
-    const C2<T> N = cdot4(pm, b0), det = cdet(pm);
+    const C2<T> det = cdet(pm);
     const C2<T> B = {k.kB * det.x, k.kB * det.y};
-    const T ay = -(k.kA * N.y), ax = k.kA * N.x;
+    const T ay = -anm.y, ax = anm.x;
     const C2<T> cof[4] = {b0[3], {-b0[2].x, -b0[2].y}, {-b0[1].x, -b0[1].y}, b0[0]};
 #pragma unroll
     for (int i = 0; i < 4; ++i)
       D[i] = drift_term(k.af, pm[i], p0[i], km[i], C2<T>{k.cs[i] - ax, ay}, pm[i], B, cof[i]);
   }
+  const C2<T> anp = drift_an(k, p0, bp);
   {
-    const C2<T> N = cdot4(p0, bp), det = cdet(bp);
+    const C2<T> det = cdet(bp);
     const C2<T> B = {k.kB * det.x, k.kB * det.y};
-    const T ax = k.kA * N.x, ay = k.kA * N.y;
+    const T ax = anp.x, ay = anp.y;
     const C2<T> cof[4] = {p0[3], {-p0[2].x, -p0[2].y}, {-p0[1].x, -p0[1].y}, p0[0]};
     if (k.conj_drift == 1) {
 #pragma unroll
@@ -238,6 +250,15 @@ __device__ __forceinline__ void drift_site(const Coef<T> &k, const C2<T> pm[4], 
       }
     }
   }
+  return anp;
+}
+
+template <typename T>
+__device__ __forceinline__ void drift_site(const Coef<T> &k, const C2<T> pm[4], const C2<T> p0[4],
+                                           const C2<T> pp[4], const C2<T> b0[4], const C2<T> bp[4],
+                                           const C2<T> km[4], const C2<T> kp[4], C2<T> D[4],
+                                           C2<T> Db[4]) {
+  (void)drift_site_carry(k, pm, p0, pp, b0, bp, km, kp, drift_an(k, pm, b0), D, Db);
 }
 
 // Component 0 of the psi drift only (what the max/mean reduction reads),
