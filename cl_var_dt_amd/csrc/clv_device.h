@@ -373,16 +373,29 @@ __host__ __device__ constexpr int row_pad(int X, int N2) {
   return p;
 }
 
+// Where the forward twiddles W_X^{n2 k1} are applied.  With short middle-level
+// lines (N2 <= 12) in the middle level, from the table entry of the inverse
+// twiddle W_X^{-m2 k1} at m2 = n2, conjugated: the first level then holds
+// only its N1 values, which keeps the fused march's register peak low.  With
+// N2 = 16 after the first level's DFT (a forward table of its own), where
+// the middle level's 2 N2 values and multipliers would need more registers.
+__host__ __device__ constexpr bool mid_twiddles(int N2) { return N2 <= 12; }
+// complex entries of the twiddle table of a split X = N1 * N2
+__host__ __device__ constexpr int tw_entries(int X, int N1) { return mid_twiddles(X / N1) ? X : 2 * X; }
+
 template <typename T, int X, int N1_ = Split<X>::N1>
 struct RowFFT {
   static constexpr int N1 = N1_, N2 = X / N1_;
   static_assert(N1 * N2 == X, "N1 must divide the row length");
   static constexpr int PAD = row_pad(X, N2);  // LDS cells per line
+  static constexpr bool MID = mid_twiddles(N2);
+  static constexpr int INV = MID ? 0 : X;     // first inverse twiddle in the table
   __device__ static __forceinline__ int pos(int p) { return p + p / N2; }
 
   // lines l < nl at buf + l*PAD; mult[k] = m(k); twiddles in the order each
   // level reads them, so the lanes of one access hit consecutive cells:
-  //   tw[k1*N2 + n2] = exp(-2 pi i n2 k1 / X),  tw[X + m2*N1 + k1] = exp(+2 pi i m2 k1 / X).
+  //   tw[INV + m2*N1 + k1] = exp(+2 pi i m2 k1 / X), and without MID also
+  //   tw[k1*N2 + n2] = exp(-2 pi i n2 k1 / X) (with MID its conjugate is used).
   // Caller syncs before (data written) and reads after the trailing sync.
   __device__ static __forceinline__ void kinetic(C2<T> *buf, const C2<T> *tw, const T *mult,
                                                  int nl, int tid, int nthr) {
@@ -393,8 +406,10 @@ struct RowFFT {
 #pragma unroll
       for (int n1 = 0; n1 < N1; ++n1) v[n1] = L[pos(n2 + N2 * n1)];
       dft<N1, -1>(v);
+      if constexpr (!MID) {
 #pragma unroll
-      for (int k1 = 1; k1 < N1; ++k1) v[k1] = cmul(v[k1], tw[k1 * N2 + n2]);
+        for (int k1 = 1; k1 < N1; ++k1) v[k1] = cmul(v[k1], tw[k1 * N2 + n2]);
+      }
 #pragma unroll
       for (int k1 = 0; k1 < N1; ++k1) L[pos(n2 + N2 * k1)] = v[k1];
     }
@@ -402,15 +417,22 @@ struct RowFFT {
     for (int task = tid; task < nl * N1; task += nthr) {
       const int l = task / N1, k1 = task - (task / N1) * N1;
       C2<T> *L = buf + l * PAD;
-      C2<T> v[N2];
+      C2<T> v[N2], w[N2];
 #pragma unroll
       for (int n2 = 0; n2 < N2; ++n2) v[n2] = L[pos(n2 + N2 * k1)];
+      if constexpr (MID) {
+#pragma unroll
+        for (int n2 = 1; n2 < N2; ++n2) {
+          w[n2] = tw[n2 * N1 + k1];
+          v[n2] = cmul(v[n2], cconj(w[n2]));
+        }
+      }
       dft<N2, -1>(v);
 #pragma unroll
       for (int k2 = 0; k2 < N2; ++k2) v[k2] = cscale(mult[k1 + N1 * k2], v[k2]);
       dft<N2, +1>(v);
 #pragma unroll
-      for (int m2 = 1; m2 < N2; ++m2) v[m2] = cmul(v[m2], tw[X + m2 * N1 + k1]);
+      for (int m2 = 1; m2 < N2; ++m2) v[m2] = cmul(v[m2], MID ? w[m2] : tw[INV + m2 * N1 + k1]);
 #pragma unroll
       for (int m2 = 0; m2 < N2; ++m2) L[pos(m2 + N2 * k1)] = v[m2];
     }

@@ -313,29 +313,39 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? kFusedWavesPerEU : 2) void k_
   // K_0 slots follow them (line 8 rows + q rows + j).
   C2<T> *const cell = sm.buf + (has_site ? 8 * j : 0) * RF::PAD + RF::pos(x);
   // Registers carried along tau (old = fields of this step, new = updated):
-  //   pm = old psi(t-1), km = K(t-1), nprev = new psi(t-1),
+  //   dc = the psi drift D(t) (formed in the previous iteration, right after
+  //   the transform that gave K(t-1)), nprev = new psi(t-1),
   //   p0 = old psi(t), b0 = old psibar(t), bp = old psibar(t+1).
   // Each iteration loads old psi(t+1) (needed after the transform) and
   // prefetches old psibar(t+2) (the next iteration's transform input), so
-  // every slice is read from memory once and the loads overlap the transform.
-  C2<T> pm[4], km[4], nprev[4], p0[4], b0[4], bp[4];
-  C2<T> an = {T(0), T(0)};  // kA N of the pair (pm, b0), carried along tau
+  // every slice is read from memory once and the loads overlap the work
+  // (issuing the prefetch after the transform instead made the step 5%
+  // slower).  Carrying D(t) instead of psi(t-1), K(t-1) and kA N keeps the
+  // register footprint small.
+  C2<T> dc[4], nprev[4], p0[4], b0[4], bp[4];
 #pragma unroll
-  for (int c = 0; c < 4; ++c) nprev[c] = {T(0), T(0)};
-  if (has_site) {
-    load4(psi, site_at(Nt - 1), pm);
-    load4(psi, site_at(0), p0);
-    load4(psib, site_at(0), b0);
-    load4(psib, site_at(1 % Nt), bp);
-#pragma unroll
-    for (int c = 0; c < 4; ++c) sm.at(c * rows + j, x) = pm[c];
+  for (int c = 0; c < 4; ++c) {
+    nprev[c] = {T(0), T(0)};
+    dc[c] = {T(0), T(0)};
   }
-  __syncthreads();
-  RF::kinetic(sm.buf, sm.tw, sm.mult, 4 * rows, tid, nthr);
-  if (has_site) {
+  {
+    C2<T> pm[4];
+    if (has_site) {
+      load4(psi, site_at(Nt - 1), pm);
+      load4(psi, site_at(0), p0);
+      load4(psib, site_at(0), b0);
+      load4(psib, site_at(1 % Nt), bp);
 #pragma unroll
-    for (int c = 0; c < 4; ++c) km[c] = sm.at(c * rows + j, x);
-    an = drift_an(k, pm, b0);
+      for (int c = 0; c < 4; ++c) sm.at(c * rows + j, x) = pm[c];
+    }
+    __syncthreads();
+    RF::kinetic(sm.buf, sm.tw, sm.mult, 4 * rows, tid, nthr);
+    if (has_site) {
+      C2<T> km[4];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) km[c] = sm.at(c * rows + j, x);
+      drift_psi(k, pm, p0, km, b0, drift_an(k, pm, b0), dc);
+    }
   }
   __syncthreads();  // the LDS lines are rewritten below
   // With two LDS sets the transform lines alternate by tau parity, so a tau's
@@ -372,7 +382,6 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? kFusedWavesPerEU : 2) void k_
     C2<T> pp[4], bn[4];
     if (has_site) {
       load4(psi, site_at(tp1), pp);
-      // old psibar(t+2) (tau = 0 at t = Nt-2)
       if (t + 2 <= Nt) load4(psib, site_at(t + 2 == Nt ? 0 : t + 2), bn);
       C2<T> *const cl = cell + L0 * RF::PAD;
 #pragma unroll
@@ -400,26 +409,35 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? kFusedWavesPerEU : 2) void k_
     }
     double v_batch = 0.0;
     if (has_site) {
-      C2<T> kn[4], kp[4];
       const C2<T> *const cl = cell + L0 * RF::PAD;
-#pragma unroll
-      for (int c = 0; c < 4; ++c) {
-        kn[c] = cl[c * RF::PAD];
-        kp[c] = cl[(4 + c) * RF::PAD];
-      }
-      C2<T> D[4], Db[4];
-      an = drift_site_carry(k, pm, p0, pp, b0, bp, km, kp, an, D, Db);
       T nz[8] = {};
       if (P.noise_on) {
         const uint32_t site = uint32_t((int64_t(t) * X + x) * A.R + r);
         normals4(nz, P.seed, 0u, step, site, 0u, sq);
         normals4(nz + 4, P.seed, 0u, step, site, 1u, sq);
       }
+      // psibar drift of this site (K of psibar(t+1) from this transform);
+      // kA N of the pair (psi(t), psibar(t+1)) is also the psi pair of t+1
+      const C2<T> an = drift_an(k, p0, bp);
       C2<T> o[4], ob[4];
-      update_site(dtT, p0, b0, D, Db, nz, o, ob);
+      {
+        C2<T> kp[4], Db[4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) kp[c] = cl[(4 + c) * RF::PAD];
+        drift_psib(k, p0, pp, b0, bp, kp, an, Db);
+        update_site(dtT, p0, b0, dc, Db, nz, o, ob);
+      }
       const int64_t s0 = site_at(t);
       store4(opsi, s0, o);
       store4(opsib, s0, ob);
+
+      // the psi drift of t+1 from K of psi(t) (this transform)
+      if (t + 1 < Nt) {
+        C2<T> kn[4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) kn[c] = cl[c * RF::PAD];
+        drift_psi(k, p0, pp, kn, bp, an, dc);
+      }
       if (t > 0) {
         const C2<T> l0 = drift0_local(k, nprev, o[0], ob);
         if (batch) v_batch = drift_abs(drift0_add_kin(l0, sm.at(L0 + (8 + B - 1) * rows + j, x)));
@@ -427,11 +445,9 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? kFusedWavesPerEU : 2) void k_
       }
 #pragma unroll
       for (int c = 0; c < 4; ++c) {
-        pm[c] = p0[c];
         p0[c] = pp[c];
         b0[c] = bp[c];
         bp[c] = bn[c];
-        km[c] = kn[c];
         nprev[c] = o[c];
       }
     }
@@ -1014,7 +1030,7 @@ std::vector<int> split_options(int X) {
 // each, plus the parked non-kinetic D_0 parts of batch - 1 taus per thread
 // and |D_0| of max(batch, 2) taus per thread for the partials.
 size_t fused_lds_bytes(const Bucket &b, int sets, int batch, int n1) {
-  return size_t(b.esz) * 2 * b.X + size_t(b.esz / 2) * (b.X + (b.X & 1)) +
+  return size_t(b.esz) * tw_entries(b.X, n1) + size_t(b.esz / 2) * (b.X + (b.X & 1)) +
          size_t(b.esz) * (8 + batch) * sets * b.rows * row_pad(b.X, b.X / n1) +
          sizeof(double) * 64 + size_t(b.esz) * (batch - 1) * b.nthr +
          sizeof(double) * (batch < 2 ? 2 : batch) * b.nthr;
@@ -1024,7 +1040,7 @@ size_t fused_lds_bytes(const Bucket &b, int sets, int batch) { return fused_lds_
 // LDS bytes of the two-pass update kernel (8 row sets) with `rows` rows per
 // tile and split n1: the largest of the two-pass and observable kernels.
 size_t update_lds_bytes(const Bucket &b, int rows, int n1) {
-  return size_t(b.esz) * 2 * b.X + size_t(b.esz / 2) * (b.X + (b.X & 1)) +
+  return size_t(b.esz) * tw_entries(b.X, n1) + size_t(b.esz / 2) * (b.X + (b.X & 1)) +
          size_t(b.esz) * 8 * rows * row_pad(b.X, b.X / n1) + sizeof(double) * 64;
 }
 
@@ -1120,19 +1136,24 @@ ChainParams to_device_params(const clv_params &p, int eval_every, int active, in
 }
 
 // Row-transform tables of lines of length L, computed in FP64 on the host:
-// twiddles in the order RowFFT reads them ([k1*N2 + n2] = exp(-2 pi i n2 k1 / L),
-// [L + m2*N1 + k1] = exp(+2 pi i m2 k1 / L)) in the field precision, and
+// twiddles in the order RowFFT reads them ([INV + m2*N1 + k1] = exp(+2 pi i m2 k1 / L),
+// and without mid_twiddles [k1*N2 + n2] = exp(-2 pi i n2 k1 / L), INV = L) in the
+// field precision, and
 // -p(k)^2 / L with p of compute_kinetic_part (Propagator_without_cutoff.cu:169-176).
 void row_tables_host(int L, int n1, int precision, std::vector<char> *tw_bytes, std::vector<double> *p2) {
   const int n1s = n1, n2s = L / n1;
-  std::vector<double> tw(4 * L);
+  const bool mid = mid_twiddles(n2s);
+  const int inv = mid ? 0 : L;
+  std::vector<double> tw(2 * tw_entries(L, n1));
   for (int k1 = 0; k1 < n1s; ++k1)
     for (int n2 = 0; n2 < n2s; ++n2) {
       const double a = 6.283185307179586 * double((n2 * k1) % L) / double(L);
-      tw[2 * (k1 * n2s + n2)] = std::cos(a);
-      tw[2 * (k1 * n2s + n2) + 1] = -std::sin(a);
-      tw[2 * (L + n2 * n1s + k1)] = std::cos(a);
-      tw[2 * (L + n2 * n1s + k1) + 1] = std::sin(a);
+      if (!mid) {
+        tw[2 * (k1 * n2s + n2)] = std::cos(a);
+        tw[2 * (k1 * n2s + n2) + 1] = -std::sin(a);
+      }
+      tw[2 * (inv + n2 * n1s + k1)] = std::cos(a);
+      tw[2 * (inv + n2 * n1s + k1) + 1] = std::sin(a);
     }
   p2->assign(L, 0.0);
   for (int k = 0; k < L; ++k) {
@@ -1273,7 +1294,7 @@ int setup_bucket(clv_ctx *ctx, Bucket &b) {
   }
   if ((rc = dalloc(ctx, b, reinterpret_cast<void **>(&b.d_prm), sizeof(ChainParams) * b.nch))) return rc;
   if ((rc = dalloc(ctx, b, reinterpret_cast<void **>(&b.d_st), 2 * sizeof(ChainState) * b.nch))) return rc;
-  if ((rc = dalloc(ctx, b, &b.d_tw, 2 * b.esz * b.X))) return rc;
+  if ((rc = dalloc(ctx, b, &b.d_tw, 2 * b.esz * b.X))) return rc;  // any split
   if ((rc = dalloc(ctx, b, reinterpret_cast<void **>(&b.d_p2), sizeof(double) * b.X))) return rc;
   if ((rc = dalloc(ctx, b, reinterpret_cast<void **>(&b.d_part), sizeof(double) * 2 * b.part_slot)))
     return rc;
@@ -2408,7 +2429,7 @@ extern "C" int clv_step_noalloc(const clv_shape *shape, int32_t precision, const
   };
   const size_t o_prm = take(sizeof(ChainParams));
   const size_t o_st = take(2 * sizeof(ChainState));
-  const size_t o_tw = take(2 * b.esz * b.X);
+  const size_t o_tw = take(size_t(b.esz) * tw_entries(b.X, b.n1));
   const size_t o_p2 = take(sizeof(double) * b.X);
   const size_t o_part = take(sizeof(double) * b.part_slot);
   if (off > kScratchBytes)

@@ -68,16 +68,15 @@ inline bool uses_fused(const Bucket &b) { return b.schedule == 2 && b.fused_lds 
 // Schedule without timing: the fused march has one workgroup per (tile, chain)
 // and walks tau serially, so it only pays off with many of them; below that the
 // two-pass kernels (one workgroup per tile and tau) fill the GPU better.  The
-// threshold is measured on MI355X: at 512 workgroups fused is 0.80-0.99 of the
-// two-pass time across 2-D and 3-D shapes and Nt = 16..128, at 320-384 it is
-// 1.05-1.33.  FP64 buckets stay on two-pass: their fused kernel spills at two
-// waves/SIMD and is slower (32 and 64 chains of 64x64x1x64: 8.5e9 against
-// 9.6e9-9.8e9 site-updates/s; 16 chains of 128x128x1x32: 7.7e9 against 8.9e9).  Both
-// schedules give identical results, so the choice may change at any step.
+// threshold is measured on MI355X: at 512 workgroups fused FP32 is 0.80-0.99 of
+// the two-pass time across 2-D and 3-D shapes and Nt = 16..128, at 320-384 it
+// is 1.05-1.33.  FP64 (two waves/SIMD, no spills): 0.80-0.81 of the two-pass
+// step at 512-1024 workgroups (32 and 64 chains of 64x64x1x64, 16 chains of
+// 128x128x1x32), equal at 256 (16 chains of 64x64x1x64).  Both schedules give
+// identical results, so the choice may change at any step.
 constexpr int kFusedMinWorkgroups = 512;
 inline int auto_schedule(const Bucket &b, int nactive) {
-  return (b.fused_lds > 0 && !b.kt_on && b.esz == 8 &&
-          int64_t(b.ntiles) * nactive >= kFusedMinWorkgroups) ? 2 : 1;
+  return (b.fused_lds > 0 && !b.kt_on && int64_t(b.ntiles) * nactive >= kFusedMinWorkgroups) ? 2 : 1;
 }
 
 }  // namespace clvi

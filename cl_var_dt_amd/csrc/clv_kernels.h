@@ -213,44 +213,44 @@ __device__ __forceinline__ C2<T> drift_an(const Coef<T> &k, const C2<T> a[4], co
   return {k.kA * N.x, k.kA * N.y};
 }
 
-// drift_site with kA N of the psi pair (pm, b0) given as anm; returns kA N of
-// the psibar pair (p0, bp), which is the next tau-slice's psi pair, so a march
-// along tau forms each N once.
+// The psi drift D (all components) from psi(t-1) = pm, psi(t) = p0, its
+// kinetic term km = K'(t-1), psibar(t) = b0 and anm = kA N of the pair (pm, b0).
 template <typename T>
-__device__ __forceinline__ C2<T> drift_site_carry(const Coef<T> &k, const C2<T> pm[4],
-                                                  const C2<T> p0[4], const C2<T> pp[4],
-                                                  const C2<T> b0[4], const C2<T> bp[4],
-                                                  const C2<T> km[4], const C2<T> kp[4], C2<T> anm,
-                                                  C2<T> D[4], C2<T> Db[4]) {
-  {
-    const C2<T> det = cdet(pm);
-    const C2<T> B = {k.kB * det.x, k.kB * det.y};
-    const T ay = -anm.y, ax = anm.x;
-    const C2<T> cof[4] = {b0[3], {-b0[2].x, -b0[2].y}, {-b0[1].x, -b0[1].y}, b0[0]};
+__device__ __forceinline__ void drift_psi(const Coef<T> &k, const C2<T> pm[4], const C2<T> p0[4],
+                                          const C2<T> km[4], const C2<T> b0[4], C2<T> anm,
+                                          C2<T> D[4]) {
+  const C2<T> det = cdet(pm);
+  const C2<T> B = {k.kB * det.x, k.kB * det.y};
+  const T ay = -anm.y, ax = anm.x;
+  const C2<T> cof[4] = {b0[3], {-b0[2].x, -b0[2].y}, {-b0[1].x, -b0[1].y}, b0[0]};
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+    D[i] = drift_term(k.af, pm[i], p0[i], km[i], C2<T>{k.cs[i] - ax, ay}, pm[i], B, cof[i]);
+}
+
+// The psibar drift Db (all components) from psibar(t+1) = bp, psibar(t) = b0,
+// its kinetic term kp = Kb'(t+1), psi(t) = p0, psi(t+1) = pp (the reference
+// conj-drift mode) and anp = kA N of the pair (p0, bp).
+template <typename T>
+__device__ __forceinline__ void drift_psib(const Coef<T> &k, const C2<T> p0[4], const C2<T> pp[4],
+                                           const C2<T> b0[4], const C2<T> bp[4],
+                                           const C2<T> kp[4], C2<T> anp, C2<T> Db[4]) {
+  const C2<T> det = cdet(bp);
+  const C2<T> B = {k.kB * det.x, k.kB * det.y};
+  const T ax = anp.x, ay = anp.y;
+  const C2<T> cof[4] = {p0[3], {-p0[2].x, -p0[2].y}, {-p0[1].x, -p0[1].y}, p0[0]};
+  if (k.conj_drift == 1) {
 #pragma unroll
     for (int i = 0; i < 4; ++i)
-      D[i] = drift_term(k.af, pm[i], p0[i], km[i], C2<T>{k.cs[i] - ax, ay}, pm[i], B, cof[i]);
-  }
-  const C2<T> anp = drift_an(k, p0, bp);
-  {
-    const C2<T> det = cdet(bp);
-    const C2<T> B = {k.kB * det.x, k.kB * det.y};
-    const T ax = anp.x, ay = anp.y;
-    const C2<T> cof[4] = {p0[3], {-p0[2].x, -p0[2].y}, {-p0[1].x, -p0[1].y}, p0[0]};
-    if (k.conj_drift == 1) {
+      Db[i] = drift_term(k.af, bp[i], b0[i], kp[i], C2<T>{k.cs[i] - ax, -ay}, bp[i], B, cof[i]);
+  } else {
 #pragma unroll
-      for (int i = 0; i < 4; ++i)
-        Db[i] = drift_term(k.af, bp[i], b0[i], kp[i], C2<T>{k.cs[i] - ax, -ay}, bp[i], B, cof[i]);
-    } else {
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        // s_i psi(t+1) - A psibar(t+1)
-        const C2<T> kin = {kp[i].x + k.cs[i] * pp[i].x, kp[i].y + k.cs[i] * pp[i].y};
-        Db[i] = drift_term(k.af, bp[i], b0[i], kin, C2<T>{-ax, -ay}, bp[i], B, cof[i]);
-      }
+    for (int i = 0; i < 4; ++i) {
+      // s_i psi(t+1) - A psibar(t+1)
+      const C2<T> kin = {kp[i].x + k.cs[i] * pp[i].x, kp[i].y + k.cs[i] * pp[i].y};
+      Db[i] = drift_term(k.af, bp[i], b0[i], kin, C2<T>{-ax, -ay}, bp[i], B, cof[i]);
     }
   }
-  return anp;
 }
 
 template <typename T>
@@ -258,7 +258,8 @@ __device__ __forceinline__ void drift_site(const Coef<T> &k, const C2<T> pm[4], 
                                            const C2<T> pp[4], const C2<T> b0[4], const C2<T> bp[4],
                                            const C2<T> km[4], const C2<T> kp[4], C2<T> D[4],
                                            C2<T> Db[4]) {
-  (void)drift_site_carry(k, pm, p0, pp, b0, bp, km, kp, drift_an(k, pm, b0), D, Db);
+  drift_psi(k, pm, p0, km, b0, drift_an(k, pm, b0), D);
+  drift_psib(k, p0, pp, b0, bp, kp, drift_an(k, p0, bp), Db);
 }
 
 // Component 0 of the psi drift only (what the max/mean reduction reads),
@@ -318,7 +319,7 @@ struct Smem {
   double *red;
   __device__ Smem(char *base, int nlines) {
     tw = reinterpret_cast<C2<T> *>(base);
-    mult = reinterpret_cast<T *>(tw + 2 * X);
+    mult = reinterpret_cast<T *>(tw + tw_entries(X, N1));
     buf = reinterpret_cast<C2<T> *>(mult + X + (X & 1));
     red = reinterpret_cast<double *>(buf + nlines * F::PAD);
   }
@@ -327,7 +328,7 @@ struct Smem {
   // first byte after the reduction scratch (extra per-kernel LDS)
   __device__ __forceinline__ C2<T> *red_end() { return reinterpret_cast<C2<T> *>(red + 64); }
   static size_t bytes(int nlines) {
-    return sizeof(C2<T>) * 2 * X + sizeof(T) * (X + (X & 1)) + sizeof(C2<T>) * nlines * F::PAD +
+    return sizeof(C2<T>) * tw_entries(X, N1) + sizeof(T) * (X + (X & 1)) + sizeof(C2<T>) * nlines * F::PAD +
            sizeof(double) * 64;
   }
 };
@@ -338,11 +339,8 @@ struct Smem {
 template <typename T, int X, int N1>
 __device__ __forceinline__ void load_tables(Smem<T, X, N1> &sm, const C2<T> *tw, const double *p2,
                                             double scale, int tid, int nthr) {
-  for (int e = tid; e < X; e += nthr) {
-    sm.tw[e] = tw[e];
-    sm.tw[X + e] = tw[X + e];
-    sm.mult[e] = T(p2[e] * scale);
-  }
+  for (int e = tid; e < tw_entries(X, N1); e += nthr) sm.tw[e] = tw[e];
+  for (int e = tid; e < X; e += nthr) sm.mult[e] = T(p2[e] * scale);
 }
 
 // Fixed-order (max, sum) of n per-tile partials, by one full wave: lane l
