@@ -46,33 +46,42 @@ def measured_traffic(kernel: str, schedule: int, lattice: str, chains: int):
     return None, None
 
 
-# MI355X vector issue: 1024 SIMDs x 16 lanes per cycle (a wave64 instruction
-# takes 4 cycles; packed FP32 off) x 2.4 GHz peak engine clock.
+# MI355X single-issue vector rate: 1024 SIMDs x 16 lanes per cycle (a wave64
+# instruction holds its SIMD for 4 cycles; packed FP32 off) x 2.4 GHz peak
+# engine clock.  gfx950 also issues two simple FP32 / 32-bit integer VALU
+# instructions from two waves in one slot, so the attainable rate for such
+# code lies above this figure (at most twice it).
 VALU_PEAK_LANE_INSTR = 1024 * 16 * 2.4e9
 
 
 def measured_valu(kernel: str, x: int, lattice: str, chains: int):
-    """VALU instructions per site-update of `kernel` (SQ_INSTS_VALU per wave
-    lane and tau slice) from the committed counter summary of this workload,
-    or None when no summary matches."""
+    """From the committed counter summary of this workload: VALU instructions
+    per site-update of `kernel` (SQ_INSTS_VALU per wave lane and tau slice),
+    the share issued in pairs and the profiled VALU busy share; (None, None)
+    when no summary matches."""
     path = os.path.join(ROOT, "profiles", f"valu_{kernel}_{lattice}_fp32.md")
     if chains != 128 or not os.path.exists(path):
         return None, None
     for line in open(path):
         cells = [c.strip() for c in line.strip().strip("|").split("|")]
-        if len(cells) >= 4 and cells[0].startswith(f"`k_{kernel}<float, {x},") and cells[3] not in ("", "-"):
-            return float(cells[3]), os.path.relpath(path, ROOT)
+        if len(cells) >= 9 and cells[0].startswith(f"`k_{kernel}<float, {x},") and cells[3] not in ("", "-"):
+            pct = lambda c: round(float(c.rstrip("%")) / 100, 2)
+            return {"instr": float(cells[3]), "pairs": pct(cells[7]), "busy": pct(cells[8])}, \
+                os.path.relpath(path, ROOT)
     return None, None
 
 
-def run_batched(args, log):
+def run_batched(args, log, fp64=False):
     import torch
-    from cl_var_dt_amd import CLV_FP32, Shape
+    from cl_var_dt_amd import CLV_FP32, CLV_FP64, Shape
     from cl_var_dt_amd.engine import Engine
 
-    X, Y, Nt, nch = args.lx, args.lx, args.nt, args.chains
+    if fp64:
+        X, Y, Nt, nch = args.fp64_lx, args.fp64_lx, args.fp64_nt, args.fp64_chains
+    else:
+        X, Y, Nt, nch = args.lx, args.lx, args.nt, args.chains
     shapes = [Shape(X, Y, 1, Nt)] * nch
-    eng = Engine(shapes, precision=CLV_FP32)
+    eng = Engine(shapes, precision=CLV_FP64 if fp64 else CLV_FP32)
     mus = [0.5 + i / max(nch - 1, 1) for i in range(nch)]
     for i in range(nch):
         eng.set_params(i, _params(1000 + i, mu=mus[i], nt=Nt))
@@ -89,7 +98,8 @@ def run_batched(args, log):
             lines = [json.loads(ln) for ln in f if ln.strip()]
         os.unlink(path)
         tuning = lines[0] if lines else None
-    log(f"batched: schedule {eng.schedule(0)}, {nch} chains of {X}x{Y}x1x{Nt} FP32, device bytes {eng.device_bytes()/1e9:.2f} GB")
+    log(f"batched: schedule {eng.schedule(0)}, {nch} chains of {X}x{Y}x1x{Nt} {'FP64' if fp64 else 'FP32'}, "
+        f"device bytes {eng.device_bytes()/1e9:.2f} GB")
     eng.propagate(args.warmup, 10)
     eng.synchronize()
     torch.cuda.synchronize()
@@ -333,6 +343,11 @@ def main() -> int:
     ap.add_argument("--lx", type=int, default=128)
     ap.add_argument("--nt", type=int, default=32)
     ap.add_argument("--schedule", type=int, default=0)
+    ap.add_argument("--fp64-chains", type=int, default=32, help="FP64 batched leg: chains (0: skip the leg)")
+    ap.add_argument("--fp64-lx", type=int, default=64)
+    ap.add_argument("--fp64-nt", type=int, default=64)
+    ap.add_argument("--precision", choices=["fp32", "fp64"], default="fp32",
+                    help="--mode batched: precision of the batched leg (fp64 uses --fp64-*)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--quiet", action="store_true")
@@ -383,10 +398,22 @@ def main() -> int:
     if args.mode == "uneven":
         print(json.dumps(run_uneven(args, log)), flush=True)
         return 0
-    b = run_batched(args, log)
+    b = run_batched(args, log, fp64=(args.mode == "batched" and args.precision == "fp64"))
     if args.mode == "batched":
         print(json.dumps({"batched_su_per_s": b["su"], "kernel_ms": b["kt"]}), flush=True)
         return 0
+    # the FP64 batched path: chains of the plain path's lattice advanced together
+    # (autotuned schedule), against 256 algorithmic bytes per site-update
+    b64 = None
+    if args.fp64_chains > 0:
+        r = run_batched(args, log, fp64=True)
+        step_s = r["wall"] / args.steps
+        b64 = {"value": round(r["su"], 1), "unit": "site-updates/s", "chains": args.fp64_chains,
+               "lattice": f"{args.fp64_lx}x{args.fp64_lx}x1x{args.fp64_nt}", "schedule": r["schedule"],
+               "ms_per_step": round(1e3 * step_s, 4),
+               "hbm_frac": round(256 * r["sites"] / step_s / 1e9 / HBM_PEAK_GBS, 4),
+               "workload": "FP64 fields and reductions, per-chain adaptive dt, observables every 10 steps, "
+                           "graph-replayed steps; hbm_frac = 256 algorithmic bytes per site-update / step time / 8 TB/s"}
     p = run_plain(args, log)
     # roofline of the dominant kernel: algorithmic bytes per launch / average launch time
     esz = 8  # complex64
@@ -410,13 +437,15 @@ def main() -> int:
             "step_frac": round(alg_per_site["fused"] * b["sites"] / (step_ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4)}
     # the issue-rate bound the fused step actually meets: its VALU instructions
     # per site-update (counter summary) at the live launch time
-    ipsu, vsrc = measured_valu(dom, args.lx, f"{args.lx}x{args.lx}x1x{args.nt}", args.chains)
+    vm, vsrc = measured_valu(dom, args.lx, f"{args.lx}x{args.lx}x1x{args.nt}", args.chains)
     valu = None
-    if ipsu:
-        ach = ipsu * b["sites"] / avg_s
-        valu = {"instr_per_site_update": ipsu, "achieved": round(ach / 1e12, 2),
-                "peak": round(VALU_PEAK_LANE_INSTR / 1e12, 2), "unit": "T lane-instr/s",
-                "frac": round(ach / VALU_PEAK_LANE_INSTR, 4), "kernel": dom, "source": vsrc}
+    if vm:
+        ach = vm["instr"] * b["sites"] / avg_s
+        valu = {"instr_per_site_update": vm["instr"], "achieved": round(ach / 1e12, 2),
+                "single_issue_peak": round(VALU_PEAK_LANE_INSTR / 1e12, 2), "unit": "T lane-instr/s",
+                "frac_of_single_issue": round(ach / VALU_PEAK_LANE_INSTR, 4),
+                "issued_in_pairs_profiled": vm["pairs"], "busy_profiled": vm["busy"],
+                "kernel": dom, "source": vsrc}
     unev = None if args.no_uneven else run_uneven(args, log)
     serv = None if args.no_serving else run_serving(args, log)
     cpu = None if args.no_cpu_baseline else run_cpu_baseline(args, log)
@@ -444,6 +473,7 @@ def main() -> int:
                        "ms_per_step": round(p["ms_per_step"], 4),
                        "workload": "one 2-D 64x64 x Nt=64 chain, FP64, host-driven dt per step"},
         "batched_over_plain": round(b["su"] / p["su"], 2),
+        "fp64_batched": b64,
         "kernel_ms": {k: round(v[0], 3) for k, v in b["kt"].items() if v[1]},
         "events_pass_su_per_s": round(b["prof_su"], 1),
         "roofline": roof,
