@@ -184,8 +184,8 @@ int clv_reset_observables(clv_ctx *ctx, int32_t chain);
  * cache_path is non-NULL the choice is read from / appended to that JSON-lines
  * file, keyed by shape, precision and device name. */
 int clv_autotune(clv_ctx *ctx, const char *cache_path);
-/* Force a schedule: 0 = auto (FP32: fused when tiles x chains >= 512
- * workgroups, else two-pass; FP64: two-pass; no timing), 1 = two-pass (reduce then update),
+/* Force a schedule: 0 = auto (fused when tiles x chains >= 512 workgroups,
+ * else two-pass; no timing), 1 = two-pass (reduce then update),
  * 2 = fused single pass (update + next step's reduction). */
 int clv_set_schedule(clv_ctx *ctx, int32_t schedule);
 /* The schedule the chain's bucket runs (1 or 2). */
