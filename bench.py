@@ -71,13 +71,13 @@ def measured_valu(kernel: str, x: int, lattice: str, chains: int):
     return None, None
 
 
-def run_batched(args, log, fp64=False):
+def run_batched(args, log, fp64=False, fp64_chains=16):
     import torch
     from cl_var_dt_amd import CLV_FP32, CLV_FP64, Shape
     from cl_var_dt_amd.engine import Engine
 
     if fp64:
-        X, Y, Nt, nch = args.fp64_lx, args.fp64_lx, args.fp64_nt, args.fp64_chains
+        X, Y, Nt, nch = args.fp64_lx, args.fp64_lx, args.fp64_nt, fp64_chains
     else:
         X, Y, Nt, nch = args.lx, args.lx, args.nt, args.chains
     shapes = [Shape(X, Y, 1, Nt)] * nch
@@ -343,7 +343,8 @@ def main() -> int:
     ap.add_argument("--lx", type=int, default=128)
     ap.add_argument("--nt", type=int, default=32)
     ap.add_argument("--schedule", type=int, default=0)
-    ap.add_argument("--fp64-chains", type=int, default=32, help="FP64 batched leg: chains (0: skip the leg)")
+    ap.add_argument("--fp64-chains", default="16,32",
+                    help="FP64 batched leg: comma-separated chain counts (0: skip the leg)")
     ap.add_argument("--fp64-lx", type=int, default=64)
     ap.add_argument("--fp64-nt", type=int, default=64)
     ap.add_argument("--precision", choices=["fp32", "fp64"], default="fp32",
@@ -398,22 +399,26 @@ def main() -> int:
     if args.mode == "uneven":
         print(json.dumps(run_uneven(args, log)), flush=True)
         return 0
-    b = run_batched(args, log, fp64=(args.mode == "batched" and args.precision == "fp64"))
+    fp64_counts = [int(c) for c in str(args.fp64_chains).split(",") if int(c) > 0]
+    b = run_batched(args, log, fp64=(args.mode == "batched" and args.precision == "fp64"),
+                    fp64_chains=fp64_counts[0] if fp64_counts else 16)
     if args.mode == "batched":
         print(json.dumps({"batched_su_per_s": b["su"], "kernel_ms": b["kt"]}), flush=True)
         return 0
     # the FP64 batched path: chains of the plain path's lattice advanced together
     # (autotuned schedule), against 256 algorithmic bytes per site-update
-    b64 = None
-    if args.fp64_chains > 0:
-        r = run_batched(args, log, fp64=True)
+    b64 = []
+    for nc in fp64_counts:
+        r = run_batched(args, log, fp64=True, fp64_chains=nc)
         step_s = r["wall"] / args.steps
-        b64 = {"value": round(r["su"], 1), "unit": "site-updates/s", "chains": args.fp64_chains,
-               "lattice": f"{args.fp64_lx}x{args.fp64_lx}x1x{args.fp64_nt}", "schedule": r["schedule"],
-               "ms_per_step": round(1e3 * step_s, 4),
-               "hbm_frac": round(256 * r["sites"] / step_s / 1e9 / HBM_PEAK_GBS, 4),
-               "workload": "FP64 fields and reductions, per-chain adaptive dt, observables every 10 steps, "
-                           "graph-replayed steps; hbm_frac = 256 algorithmic bytes per site-update / step time / 8 TB/s"}
+        t = r["tuning"] or {}
+        b64.append({"value": round(r["su"], 1), "unit": "site-updates/s", "chains": nc,
+                    "lattice": f"{args.fp64_lx}x{args.fp64_lx}x1x{args.fp64_nt}", "schedule": r["schedule"],
+                    "tau_blocks": t.get("fblocks"), "ms_per_step": round(1e3 * step_s, 4),
+                    "hbm_frac": round(256 * r["sites"] / step_s / 1e9 / HBM_PEAK_GBS, 4),
+                    "workload": "FP64 fields and reductions, per-chain adaptive dt, observables every 10 steps, "
+                                "autotuned, graph-replayed steps; hbm_frac = 256 algorithmic bytes per "
+                                "site-update / step time / 8 TB/s"})
     p = run_plain(args, log)
     # roofline of the dominant kernel: algorithmic bytes per launch / average launch time
     esz = 8  # complex64
@@ -473,7 +478,7 @@ def main() -> int:
                        "ms_per_step": round(p["ms_per_step"], 4),
                        "workload": "one 2-D 64x64 x Nt=64 chain, FP64, host-driven dt per step"},
         "batched_over_plain": round(b["su"] / p["su"], 2),
-        "fp64_batched": b64,
+        "fp64_batched": b64 or None,
         "kernel_ms": {k: round(v[0], 3) for k, v in b["kt"].items() if v[1]},
         "events_pass_su_per_s": round(b["prof_su"], 1),
         "roofline": roof,

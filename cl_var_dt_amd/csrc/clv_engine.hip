@@ -46,11 +46,14 @@ __global__ __launch_bounds__(256) void k_drift_stats(BucketArgs<T> A) {
   const int chain = A.chain0 + blockIdx.z;
   const ChainParams &P = A.prm[chain];
   if (!P.active) return;
-  const int tile = blockIdx.x, t0 = blockIdx.y * A.tcs;
+  // TC slices from t0; with a stride, one slice per workgroup (the first tau
+  // of each block of the tau-blocked fused march)
+  const int TC = A.tstride ? 1 : A.tcs;
+  const int tile = blockIdx.x, t0 = blockIdx.y * (A.tstride ? A.tstride : TC);
   const int tid = threadIdx.x, nthr = blockDim.x;
   const int j = tid / X, x = tid - (tid / X) * X;
   const bool has_site = j < A.rows;
-  const int rows = A.rows, TC = A.tcs;
+  const int rows = A.rows;
   Smem<T, X, N1> sm(smem_raw, TC * rows);
   double *vb = reinterpret_cast<double *>(sm.red_end());  // [TC][nthr] |D_0|
   load_tables<T, X, N1>(sm, A.tw, A.p2, drift_table_scale(P), tid, nthr);
@@ -254,8 +257,10 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? kFusedWavesPerEU : 2) void k_
   const int tid = threadIdx.x, nthr = blockDim.x;
   const ChainParams &P = A.prm[chain];
   const ChainState &st = A.st[chain];
+  // tau block of this workgroup: [t0, t0 + Lb)
+  const int Lb = A.Nt / A.fblocks, t0 = int(blockIdx.z) * Lb, t1 = t0 + Lb;
   if (!P.active) {
-    if (tile == 0 && tid == 0) A.st_next[chain] = st;
+    if (tile == 0 && t0 == 0 && tid == 0) A.st_next[chain] = st;
     return;
   }
   const int rows = A.rows, Nt = A.Nt, B = A.fbatch;
@@ -276,7 +281,7 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? kFusedWavesPerEU : 2) void k_
       int clamp = 0;
       const double dt = new_dt(mean, m, P, &clamp);
       sm.red[0] = dt;
-      if (tile == 0) {
+      if (tile == 0 && t0 == 0) {
         ChainState ns = st;
         ns.dt = dt;
         ns.max_drift = m;
@@ -331,10 +336,10 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? kFusedWavesPerEU : 2) void k_
   {
     C2<T> pm[4];
     if (has_site) {
-      load4(psi, site_at(Nt - 1), pm);
-      load4(psi, site_at(0), p0);
-      load4(psib, site_at(0), b0);
-      load4(psib, site_at(1 % Nt), bp);
+      load4(psi, site_at(t0 == 0 ? Nt - 1 : t0 - 1), pm);
+      load4(psi, site_at(t0), p0);
+      load4(psib, site_at(t0), b0);
+      load4(psib, site_at((t0 + 1) % Nt), bp);
 #pragma unroll
       for (int c = 0; c < 4; ++c) sm.at(c * rows + j, x) = pm[c];
     }
@@ -350,39 +355,43 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? kFusedWavesPerEU : 2) void k_
   __syncthreads();  // the LDS lines are rewritten below
   // With two LDS sets the transform lines alternate by tau parity, so a tau's
   // lines can be written while slower waves still read the previous tau's
-  // results (one set: a barrier ends each tau).  The batch taus (t % B == B-1)
-  // are odd for B > 1, so the parked slots sit at the end of set (B > 1) * (fsets - 1).
-  // Each tau's |D_0| go to LDS slot tau % max(B, 2); after the next barrier
+  // results (one set: a barrier ends each tau).  Taus are counted from the
+  // block's first, u = t - t0.  The batch taus (u % B == B-1) are odd for
+  // B > 1, so the parked slots sit at the end of set (B > 1) * (fsets - 1).
+  // Each tau's |D_0| go to LDS slot u % max(B, 2); after the next barrier
   // one wave per tau reduces them (tile_max_sum) and writes the partial, the
-  // B taus of a batch spread over the waves.
+  // B taus of a batch spread over the waves.  The block's first tau has no
+  // |D_0| here (it needs the new psi of the tau before, another block's):
+  // with one block the march ends with it, else k_drift_stats adds it.
   const int w = tid >> 6, nw = nthr >> 6, lane = tid & 63;
   const int rmask = (B < 2 ? 2 : B) - 1;
   const int park = (B > 1 ? A.fsets - 1 : 0) * setw + 8 * rows;  // first parked slot (B > 1)
-  auto put_v = [&](int tau, double v) { vb[(tau & rmask) * nthr + tid] = v; };
-  auto partial = [&](int tau, int slot) {  // one whole wave, after a barrier
+  auto put_v = [&](int u, double v) { vb[(u & rmask) * nthr + tid] = v; };
+  auto partial = [&](int u, int slot) {  // one whole wave, after a barrier
     double m, s;
     tile_max_sum(vb + slot * nthr, nthr, lane, &m, &s);
     if (lane == 0) {
-      double *o = A.part_wr + ((int64_t(chain) * Nt + tau) * A.ntiles + tile) * 2;
+      double *o = A.part_wr + ((int64_t(chain) * Nt + t0 + u) * A.ntiles + tile) * 2;
       o[0] = m;
       o[1] = s;
     }
   };
-  // the partials of batch tau tb: taus max(1, tb-B+1) .. tb, q-th by wave q % nw
-  auto reduce_batch = [&](int tb) {
+  // the partials of batch tau ub: taus max(1, ub-B+1) .. ub, q-th by wave q % nw
+  auto reduce_batch = [&](int ub) {
     for (int q = w; q < B; q += nw) {
-      const int tau = tb - B + 1 + q;
-      if (tau >= 1) partial(tau, tau & rmask);
+      const int u = ub - B + 1 + q;
+      if (u >= 1) partial(u, u & rmask);
     }
   };
-  for (int t = 0; t < Nt; ++t) {
+  for (int t = t0; t < t1; ++t) {
+    const int u = t - t0;
     const int tp1 = (t == Nt - 1) ? 0 : t + 1;
-    const int L0 = (t & (A.fsets - 1)) * setw;  // first line of this tau's set
-    const bool batch = t > 0 && (t & (B - 1)) == B - 1;
+    const int L0 = (u & (A.fsets - 1)) * setw;  // first line of this tau's set
+    const bool batch = u > 0 && (u & (B - 1)) == B - 1;
     C2<T> pp[4], bn[4];
     if (has_site) {
       load4(psi, site_at(tp1), pp);
-      if (t + 2 <= Nt) load4(psib, site_at(t + 2 == Nt ? 0 : t + 2), bn);
+      if (t + 1 < t1) load4(psib, site_at((t + 2) % Nt), bn);
       C2<T> *const cl = cell + L0 * RF::PAD;
 #pragma unroll
       for (int c = 0; c < 4; ++c) {
@@ -394,17 +403,17 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? kFusedWavesPerEU : 2) void k_
       if (batch) sm.at(L0 + (8 + B - 1) * rows + j, x) = nprev[0];
     }
     __syncthreads();
-    if (t > 1 && ((t - 1) & (B - 1)) == B - 1) reduce_batch(t - 1);
-    if (has_site && !batch && t > 0) sm.at(park + (t & (B - 1)) * rows + j, x) = nprev[0];
+    if (u > 1 && ((u - 1) & (B - 1)) == B - 1) reduce_batch(u - 1);
+    if (has_site && !batch && u > 0) sm.at(park + (u & (B - 1)) * rows + j, x) = nprev[0];
     RF::kinetic(sm.buf + L0 * RF::PAD, sm.tw, sm.mult, (batch ? 8 + B : 8) * rows, tid, nthr);
     // the parked taus of this batch
     if (batch) {
       for (int q = 0; q < B - 1; ++q) {
-        const int tau = t - (B - 1) + q;
-        if (tau < 1) continue;
+        const int uq = u - (B - 1) + q;
+        if (uq < 1) continue;
         double v = 0.0;
         if (has_site) v = drift_abs(drift0_add_kin(lpark[q * nthr + tid], sm.at(L0 + (8 + q) * rows + j, x)));
-        put_v(tau, v);
+        put_v(uq, v);
       }
     }
     double v_batch = 0.0;
@@ -432,16 +441,16 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? kFusedWavesPerEU : 2) void k_
       store4(opsib, s0, ob);
 
       // the psi drift of t+1 from K of psi(t) (this transform)
-      if (t + 1 < Nt) {
+      if (t + 1 < t1) {
         C2<T> kn[4];
 #pragma unroll
         for (int c = 0; c < 4; ++c) kn[c] = cl[c * RF::PAD];
         drift_psi(k, p0, pp, kn, bp, an, dc);
       }
-      if (t > 0) {
+      if (u > 0) {
         const C2<T> l0 = drift0_local(k, nprev, o[0], ob);
         if (batch) v_batch = drift_abs(drift0_add_kin(l0, sm.at(L0 + (8 + B - 1) * rows + j, x)));
-        else lpark[(t & (B - 1)) * nthr + tid] = l0;
+        else lpark[(u & (B - 1)) * nthr + tid] = l0;
       }
 #pragma unroll
       for (int c = 0; c < 4; ++c) {
@@ -451,44 +460,48 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? kFusedWavesPerEU : 2) void k_
         nprev[c] = o[c];
       }
     }
-    if (batch) put_v(t, v_batch);
+    if (batch) put_v(u, v_batch);
     if (A.fsets == 1) __syncthreads();  // the one set is rewritten by the next tau
   }
   __syncthreads();
-  const int tl = Nt - 1;  // last tau of the march
-  const bool last_batch = tl > 0 && (tl & (B - 1)) == B - 1;
-  if (last_batch) reduce_batch(tl);
+  const int ul = Lb - 1;  // last tau of the march
+  const bool last_batch = ul > 0 && (ul & (B - 1)) == B - 1;
+  if (last_batch) reduce_batch(ul);
   // K_0 lines still to transform: the taus parked after the last batch
-  // (slots 0 .. np-1; slot 0 unused before the first batch) and tau = 0
-  // (new psi_0(Nt-1)) in the slot after them.  B = 1 uses line 0.
-  const int np = (last_batch || tl == 0) ? 0 : (tl & (B - 1)) + 1;
+  // (slots 0 .. np-1; slot 0 unused before the first batch) and, with one
+  // block, tau = 0 (new psi_0(Nt-1)) in the slot after them.  B = 1 uses line 0.
+  const int np = (last_batch || ul == 0) ? 0 : (ul & (B - 1)) + 1;
+  const int nz0 = (A.fblocks == 1) ? 1 : 0;  // the tau = 0 item
+  if (np + nz0 == 0) return;                 // uniform: nothing pending
   const int pb = (B > 1) ? park : 0;
   // |D_0| of the new tau = 0 (needs the new psi(Nt-1)); this thread wrote
   // the new tau = 0 values of its site at t = 0
   C2<T> n0k[4], nb0k[4];
-  if (has_site) {
+  if (has_site && nz0) {
     load4(opsi, site_at(0), n0k);
     load4(opsib, site_at(0), nb0k);
     sm.at(pb + np * rows + j, x) = nprev[0];
   }
   __syncthreads();
-  RF::kinetic(sm.buf + pb * RF::PAD, sm.tw, sm.mult, (np + 1) * rows, tid, nthr);
-  // the pending taus in slots 0 .. np-1 (the slot of tau0 + q is q), tau = 0
-  // in slot np; items i = 0 .. np, item i reduced by wave i % nw
-  const int tau0 = tl - (np - 1);
+  RF::kinetic(sm.buf + pb * RF::PAD, sm.tw, sm.mult, (np + nz0) * rows, tid, nthr);
+  // the pending taus in slots 0 .. np-1 (the slot of u0 + q is q), tau = 0
+  // in slot np; items i = 0 .. np + nz0 - 1, item i reduced by wave i % nw
+  const int u0 = ul - (np - 1);
   for (int q = 0; q < np; ++q) {
-    if (tau0 + q < 1) continue;
+    if (u0 + q < 1) continue;
     double v = 0.0;
     if (has_site) v = drift_abs(drift0_add_kin(lpark[q * nthr + tid], sm.at(pb + q * rows + j, x)));
     vb[q * nthr + tid] = v;
   }
-  double v0 = 0.0;
-  if (has_site) v0 = drift_abs(drift0_site(k, nprev, n0k[0], nb0k, sm.at(pb + np * rows + j, x)));
-  vb[np * nthr + tid] = v0;
+  if (nz0) {
+    double v0 = 0.0;
+    if (has_site) v0 = drift_abs(drift0_site(k, nprev, n0k[0], nb0k, sm.at(pb + np * rows + j, x)));
+    vb[np * nthr + tid] = v0;
+  }
   __syncthreads();
-  for (int i = w; i <= np; i += nw) {
-    if (i < np && tau0 + i < 1) continue;
-    partial(i < np ? tau0 + i : 0, i);
+  for (int i = w; i < np + nz0; i += nw) {
+    if (i < np && u0 + i < 1) continue;
+    partial(i < np ? u0 + i : 0, i);
   }
 }
 
@@ -797,13 +810,16 @@ BucketArgs<T> make_args(const Bucket &b, int chain0, int eval_every, bool fused 
   A.tcs = b.tcs;
   A.fsets = b.fused_sets;
   A.fbatch = b.fused_batch;
+  A.fblocks = b.fused_blocks;
+  A.tstride = 0;
   A.chain0 = chain0;
   A.eval_every = eval_every;
   return A;
 }
 
 // ---- launchers, dispatched on X at run time ----
-// which: 0 drift_stats, 1 update, 2 obs, 3 fused
+// which: 0 drift_stats, 1 update, 2 obs, 3 fused, 4 drift stats of the fused
+// march's block boundaries
 template <typename T, int X, int N1>
 void launch_row_kernels(const Bucket &b, hipStream_t s, int which, int chain0, int nch,
                         int eval_every) {
@@ -820,9 +836,16 @@ void launch_row_kernels(const Bucket &b, hipStream_t s, int which, int chain0, i
     BucketArgs<T> A = make_args<T>(b, chain0, eval_every);
     const size_t sh = Smem<T, X, N1>::bytes(4 * obs_slices(int(sizeof(C2<T>)), X) * b.rows);
     hipLaunchKernelGGL((k_obs<T, X, N1>), dim3(b.ntiles, b.nchunks, nch), block, sh, s, A);
-  } else {
+  } else if (which == 3) {
     BucketArgs<T> A = make_args<T>(b, chain0, eval_every, true);
-    hipLaunchKernelGGL((k_fused<T, X, N1>), dim3(b.ntiles, nch), block, b.fused_lds, s, A);
+    hipLaunchKernelGGL((k_fused<T, X, N1>), dim3(b.ntiles, nch, b.fused_blocks), block, b.fused_lds, s, A);
+  } else {
+    // |D_0| partials of the first tau of each block of a tau-blocked fused
+    // march (the new fields: launched after the state slots swap)
+    BucketArgs<T> A = make_args<T>(b, chain0, eval_every);
+    A.tstride = b.Nt / b.fused_blocks;
+    const size_t sh = Smem<T, X, N1>::bytes(b.rows) + sizeof(double) * b.nthr;
+    hipLaunchKernelGGL((k_drift_stats<T, X, N1>), dim3(b.ntiles, b.fused_blocks, nch), block, sh, s, A);
   }
 }
 
@@ -969,7 +992,7 @@ void drain_events(clv_ctx *c) {
 
 // precision-dispatching wrappers
 void row_kernel(clv_ctx *c, const Bucket &b, int which, int chain0, int nch, int ev) {
-  static const int kinds[4] = {0, 2, 3, 5};
+  static const int kinds[5] = {0, 2, 3, 5, 0};
   timed(c, kinds[which], [&] {
     if (c->precision == CLV_FP32) dispatch_row<float>(b, c->stream, which, chain0, nch, ev);
     else dispatch_row<double>(b, c->stream, which, chain0, nch, ev);
@@ -1250,7 +1273,7 @@ void plan_bucket(Bucket &b, int precision) {
     }
   b.fused_lds = fused_lds_bytes(b, b.fused_sets, b.fused_batch);
   if (b.fused_lds > 65536) b.fused_lds = 0;
-  b.schedule = auto_schedule(b, b.nch);
+  auto_plan(b, b.nch);
 }
 
 // Switch bucket b to row split n1 (0: the default): new twiddle tables, the
@@ -1345,6 +1368,7 @@ void step_bucket(clv_ctx *ctx, Bucket &b, int eval_every) {
     if (!b.primed) row_kernel(ctx, b, 0, 0, b.nch, 0);  // partials of the current fields
     row_kernel(ctx, b, 3, 0, b.nch, 0);
     b.epoch ^= 1;  // new state and next step's partials live in the other slot
+    if (b.fused_blocks > 1) row_kernel(ctx, b, 4, 0, b.nch, 0);
     b.primed = true;
   } else {
     transverse(ctx, b, 0, b.nch, 0, 0);
@@ -1366,6 +1390,7 @@ void enqueue_step_gated(clv_ctx *ctx, Bucket &b, int eval_every, bool obs) {
   if (uses_fused(b)) {
     row_kernel(ctx, b, 3, 0, b.nch, 0);
     b.epoch ^= 1;
+    if (b.fused_blocks > 1) row_kernel(ctx, b, 4, 0, b.nch, 0);
   } else {
     transverse(ctx, b, 0, b.nch, 0, 0);
     row_kernel(ctx, b, 0, 0, b.nch, 0);
@@ -1781,7 +1806,8 @@ static int graph_for(clv_ctx *ctx, int eval_every, hipGraphExec_t *out, int *ste
   std::string key = std::to_string(eval_every) + (phased ? "p" : "g");
   for (size_t i = 0; i < ctx->buckets.size(); ++i) {
     const Bucket &b = ctx->buckets[i];
-    key += "/" + std::to_string(b.schedule) + ":" + std::to_string(b.epoch & 1);
+    key += "/" + std::to_string(b.schedule) + "." + std::to_string(b.fused_blocks) + ":" +
+           std::to_string(b.epoch & 1);
     if (phased) key += ":" + std::to_string(phase[i]);
   }
   *steps = G;
@@ -2042,7 +2068,7 @@ std::string tune_key(const std::string &dev, const Bucket &b, int precision) {
 // as cufftPlanMany's own plan choice, Lattice.cu:135-156).  The row tile is
 // fixed per shape: it sets the partial-sum granularity.
 struct TuneChoice {
-  int schedule = 1, tcs = 1, fsets = 1, fbatch = 1, n1 = 0;
+  int schedule = 1, tcs = 1, fsets = 1, fbatch = 1, n1 = 0, fblocks = 1;
 };
 
 int json_int(const std::string &line, const char *field, int dflt) {
@@ -2051,8 +2077,8 @@ int json_int(const std::string &line, const char *field, int dflt) {
   return p == std::string::npos ? dflt : std::atoi(line.c_str() + p + k.size());
 }
 
-// cache lines: {"key": "...", "schedule": S, "tcs": T, "fsets": F, "fbatch": B, "n1": N, "ms": best,
-//               ..., "candidates": [[schedule, tcs, fsets, fbatch, n1, ms], ...]}
+// cache lines: {"key": "...", "schedule": S, "tcs": T, "fsets": F, "fbatch": B, "n1": N,
+//               "fblocks": K, "ms": best, ..., "candidates": [[schedule, tcs, fsets, fbatch, n1, fblocks, ms], ...]}
 std::map<std::string, TuneChoice> read_tune_cache(const char *path) {
   std::map<std::string, TuneChoice> out;
   if (!path) return out;
@@ -2068,6 +2094,7 @@ std::map<std::string, TuneChoice> read_tune_cache(const char *path) {
     c.fsets = json_int(line, "fsets", 0);
     c.fbatch = json_int(line, "fbatch", 1);
     c.n1 = json_int(line, "n1", 0);
+    c.fblocks = json_int(line, "fblocks", 1);
     out[line.substr(k0 + 8, k1 - k0 - 8)] = c;
   }
   return out;
@@ -2127,7 +2154,8 @@ int restore_snapshot(clv_ctx *ctx, Bucket &b, Snapshot &sn, bool release) {
 // Valid choices for bucket b: every instantiated row split whose kernels fit
 // 64 KB of LDS at the bucket's row tile, and for each the reduction tiles that
 // divide Nt and fit LDS, and the fused line sets and K_0 batches (of at most
-// Nt) that fit 64 KB.
+// Nt) that fit 64 KB, each with one tau block and, where the chains give
+// fewer than kFusedMinWorkgroups workgroups, with the untimed block count.
 std::vector<TuneChoice> tune_choices(const Bucket &b) {
   std::vector<TuneChoice> out;
   for (int n1 : split_options(b.X)) {
@@ -2139,7 +2167,11 @@ std::vector<TuneChoice> tune_choices(const Bucket &b) {
     if (!b.kt_on)
       for (int f = 1; f <= 2; ++f)
         for (int B = 1; B <= 4; B *= 2)
-          if ((B == 1 || B <= b.Nt) && fused_lds_bytes(b, f, B, n1) <= 65536) out.push_back({2, b.tcs, f, B, n1});
+          if ((B == 1 || B <= b.Nt) && fused_lds_bytes(b, f, B, n1) <= 65536) {
+            out.push_back({2, b.tcs, f, B, n1, 1});
+            const int nb = fused_blocks_for(b, b.nch);
+            if (nb > 1) out.push_back({2, b.tcs, f, B, n1, nb});
+          }
   }
   return out;
 }
@@ -2147,7 +2179,7 @@ std::vector<TuneChoice> tune_choices(const Bucket &b) {
 bool choice_valid(const Bucket &b, const TuneChoice &c) {
   for (const TuneChoice &v : tune_choices(b))
     if (v.n1 == c.n1 && v.schedule == c.schedule && (c.schedule == 2 || v.tcs == c.tcs) &&
-        (c.schedule == 1 || (v.fsets == c.fsets && v.fbatch == c.fbatch)))
+        (c.schedule == 1 || (v.fsets == c.fsets && v.fbatch == c.fbatch && v.fblocks == c.fblocks)))
       return true;
   return false;
 }
@@ -2163,6 +2195,7 @@ int apply_choice(clv_ctx *ctx, Bucket &b, const TuneChoice &c) {
   } else {
     b.fused_sets = c.fsets;
     b.fused_batch = c.fbatch;
+    b.fused_blocks = c.fblocks;
     b.fused_lds = fused_lds_bytes(b, c.fsets, c.fbatch);
   }
   b.primed = false;
@@ -2239,12 +2272,12 @@ int clv_autotune(clv_ctx *ctx, const char *cache_path) {
       std::ofstream out(cache_path, std::ios::app);
       out << "{\"key\": \"" << key << "\", \"schedule\": " << cand[best].schedule
           << ", \"tcs\": " << b.tcs << ", \"fsets\": " << b.fused_sets << ", \"fbatch\": " << b.fused_batch
-          << ", \"n1\": " << b.n1
+          << ", \"n1\": " << b.n1 << ", \"fblocks\": " << b.fused_blocks
           << ", \"ms\": " << ms[best]
           << ", \"ms_two_pass\": " << two << ", \"ms_fused\": " << fused << ", \"candidates\": [";
       for (size_t q = 0; q < cand.size(); ++q)
         out << (q ? ", " : "") << "[" << cand[q].schedule << ", " << cand[q].tcs << ", " << cand[q].fsets
-            << ", " << cand[q].fbatch << ", " << cand[q].n1 << ", " << ms[q] << "]";
+            << ", " << cand[q].fbatch << ", " << cand[q].n1 << ", " << cand[q].fblocks << ", " << ms[q] << "]";
       out << "]}\n";
     }
   }
@@ -2264,7 +2297,26 @@ int clv_set_schedule(clv_ctx *ctx, int32_t schedule) {
     if (schedule == 2 && b.kt_on)
       return fail(ctx, CLV_ERR_ARG, "clv_set_schedule: chains with kin_dims = 3 run the two-pass schedule");
   }
-  for (Bucket &b : ctx->buckets) b.schedule = (schedule == 0) ? auto_schedule(b, b.nch) : schedule;
+  for (Bucket &b : ctx->buckets) {
+    if (schedule == 0) auto_plan(b, b.nch);
+    else b.schedule = schedule;
+  }
+  drop_graphs(ctx);
+  return CLV_OK;
+}
+
+int clv_set_fused_blocks(clv_ctx *ctx, int32_t blocks) {
+  if (!ctx) return CLV_ERR_ARG;
+  if (blocks < 0) return fail(ctx, CLV_ERR_ARG, "clv_set_fused_blocks: blocks < 0");
+  for (const Bucket &b : ctx->buckets)
+    if (blocks > 0 && (b.Nt % blocks != 0 || b.Nt / blocks < 2))
+      return fail(ctx, CLV_ERR_ARG, "clv_set_fused_blocks: " + std::to_string(blocks) +
+                                        " blocks do not divide Nt = " + std::to_string(b.Nt) +
+                                        " into blocks of at least 2 slices");
+  for (Bucket &b : ctx->buckets) {
+    const int nb = (blocks > 0) ? blocks : fused_blocks_for(b, b.nch);
+    b.fused_blocks = nb > 0 ? nb : 1;
+  }
   drop_graphs(ctx);
   return CLV_OK;
 }

@@ -49,6 +49,7 @@ struct Bucket {
   int fused_sets = 1;    // LDS line sets of the fused kernel (2 when they fit)
   int fused_sets_max = 1;
   int fused_batch = 1;   // tau slices per batch of next-step K_0 lines (1, 2, 4)
+  int fused_blocks = 1;  // tau blocks of the fused march (a divisor of Nt, blocks of >= 2 slices)
   // kin_dims = 3: transverse kinetic term kt (allocated when a chain asks for it)
   struct KtAxis {
     int L = 1, bmul = 0, rstride = 0, nbase = 0, W = 1;
@@ -75,8 +76,24 @@ inline bool uses_fused(const Bucket &b) { return b.schedule == 2 && b.fused_lds 
 // 128x128x1x32), equal at 256 (16 chains of 64x64x1x64).  Both schedules give
 // identical results, so the choice may change at any step.
 constexpr int kFusedMinWorkgroups = 512;
+// Tau blocks of the fused march for nactive chains: the fewest (a power of two
+// dividing Nt, blocks of at least kFusedMinBlock slices) that reach
+// kFusedMinWorkgroups workgroups, 0 when none does.
+constexpr int kFusedMinBlock = 8;
+inline int fused_blocks_for(const Bucket &b, int nactive) {
+  for (int nb = 1; nb <= b.Nt / kFusedMinBlock || nb == 1; nb *= 2) {
+    if (b.Nt % nb) break;
+    if (int64_t(b.ntiles) * nactive * nb >= kFusedMinWorkgroups) return nb;
+  }
+  return 0;
+}
 inline int auto_schedule(const Bucket &b, int nactive) {
-  return (b.fused_lds > 0 && !b.kt_on && int64_t(b.ntiles) * nactive >= kFusedMinWorkgroups) ? 2 : 1;
+  return (b.fused_lds > 0 && !b.kt_on && fused_blocks_for(b, nactive) > 0) ? 2 : 1;
+}
+// Untimed choice: the schedule and, for the fused one, its tau blocks.
+inline void auto_plan(Bucket &b, int nactive) {
+  b.schedule = auto_schedule(b, nactive);
+  b.fused_blocks = (b.schedule == 2) ? fused_blocks_for(b, nactive) : 1;
 }
 
 }  // namespace clvi

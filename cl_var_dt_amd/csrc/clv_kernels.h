@@ -35,6 +35,8 @@ struct BucketArgs {
   int tcs;                      // tau slices per drift-stats workgroup
   int fsets;                    // fused kernel: LDS line sets (2: alternate by tau parity)
   int fbatch;                   // fused kernel: tau slices per batch of next-step K_0 lines (1, 2, 4)
+  int fblocks;                  // fused kernel: tau blocks (one workgroup each) per (tile, chain)
+  int tstride;                  // drift stats: > 0 processes one slice per workgroup, at blockIdx.y * tstride
   int chain0;                  // first chain of the launch
   int eval_every;               // obs kernels: evaluate chains with cur_step % eval_every == 0;
                                 // < 0: each chain's own ChainParams::eval_every; 0: always

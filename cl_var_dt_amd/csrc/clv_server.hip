@@ -418,7 +418,7 @@ void worker_main(clv_server *s) {
       for (Run *r : p.slot_run)
         if (r) k = std::min(k, r->remaining);
       Bucket &b = p.ctx.buckets[0];
-      b.schedule = auto_schedule(b, p.nactive);
+      auto_plan(b, p.nactive);
       (void)hipEventRecord(p.ev_start, p.ctx.stream);
       for (int64_t n = 0; n < k; ++n) step_bucket(&p.ctx, b, -1);
       (void)hipEventRecord(p.ev_done, p.ctx.stream);

@@ -153,6 +153,10 @@ class Engine:
         """Fused schedule: LDS line sets (1, 2) and K_0 batch (1, 2, 4); 0 keeps."""
         self._chk(lib().clv_set_fused_options(self._h, line_sets, batch))
 
+    def set_fused_blocks(self, blocks: int = 0) -> None:
+        """Fused schedule: tau blocks per (row tile, chain); 0 = untimed choice."""
+        self._chk(lib().clv_set_fused_blocks(self._h, blocks))
+
     def set_graphs(self, on: bool) -> None:
         self._chk(lib().clv_set_graphs(self._h, 1 if on else 0))
 

@@ -208,6 +208,15 @@ int clv_get_row_split(const clv_ctx *ctx, int32_t chain);
  * Results do not depend on either; a combination that does not fit 64 KB of
  * LDS for some shape is rejected.  clv_autotune times them. */
 int clv_set_fused_options(clv_ctx *ctx, int32_t line_sets, int32_t batch);
+/* Tau blocks of the fused schedule for every bucket (project extension, no
+ * counterpart in the reference): the tau range splits into `blocks` equal
+ * blocks, one workgroup per (row tile, chain, block), so few chains still
+ * give enough workgroups; the first slice of each block gets its next-step
+ * |D_0| partial from a short drift-statistics pass after the march.
+ * blocks must divide Nt into blocks of at least 2 slices; 0 = the untimed
+ * choice (the fewest blocks of at least 8 slices that give 512 workgroups).
+ * Results do not depend on it; clv_autotune times it. */
+int clv_set_fused_blocks(clv_ctx *ctx, int32_t blocks);
 /* clv_propagate replays captured HIP graphs of its steps (on by default): a
  * cadence period with the observable kernels only where they evaluate when
  * eval_every > 0 and each bucket's chains share a step count, else two steps

@@ -405,6 +405,66 @@ def test_fused_options_bit_identical_to_two_pass(prec):
 
 
 @pytest.mark.parametrize("prec", [CLV_FP32, CLV_FP64])
+def test_fused_tau_blocks_bit_identical_to_two_pass(prec):
+    """The tau-blocked fused march (one workgroup per row tile, chain and
+    block of Nt / blocks slices, the first slice of each block completed by a
+    drift-statistics pass) reproduces the two-pass schedule bit for bit, in
+    eager steps and graph replay, for block counts 2..Nt/2 and every K_0 batch
+    (blocks shorter than, equal to and longer than the batch), with the
+    untimed block choice and through a change of block count mid-run."""
+    print("case: fused tau blocks", prec, flush=True)
+    from cl_var_dt_amd._abi import SCHED_FUSED, SCHED_TWO_PASS
+    shapes = [(128, 2, 1, 32), (64, 4, 1, 8), (16, 8, 2, 12), (32, 2, 1, 16)]
+    if prec == CLV_FP64:
+        shapes = [(32, 2, 1, 8), (16, 4, 1, 12), (64, 2, 1, 8)]
+
+    def run(sched, blocks=None, batch=0, graphs=False, switch=None):
+        e = _engine(shapes, prec)
+        for i in range(len(shapes)):
+            e.set_params(i, _params(seed=70 + i, mu=0.6 + 0.1 * i))
+            e.init_fields(i, 1.0, 0.1)
+        e.set_schedule(sched)
+        e.set_graphs(graphs)
+        if batch:
+            e.set_fused_options(0, batch)
+        if blocks is not None:
+            e.set_fused_blocks(blocks)
+        e.propagate(7, 3)
+        if switch is not None:
+            e.set_fused_blocks(switch)
+        e.propagate(6, 3)
+        return e
+
+    def same(ref, e, label):
+        for i in range(len(shapes)):
+            fa, ga = ref.get_fields(i)
+            fb, gb = e.get_fields(i)
+            assert np.array_equal(fa, fb) and np.array_equal(ga, gb), (label, shapes[i])
+            sa, sb = ref.stats(i), e.stats(i)
+            assert (sa.dt, sa.max_drift, sa.mean_drift, sa.t_langevin) == \
+                   (sb.dt, sb.max_drift, sb.mean_drift, sb.t_langevin), (label, shapes[i])
+            oa, _ = ref.observables(i)
+            ob, _ = e.observables(i)
+            assert oa.n_tot == ob.n_tot and oa.energy == ob.energy, (label, shapes[i])
+
+    ref = run(SCHED_TWO_PASS)
+    # every block count dividing all Nt into blocks of >= 2 slices
+    nts = [s[3] for s in shapes]
+    counts = [nb for nb in range(2, min(nts) // 2 + 1) if all(n % nb == 0 for n in nts)]
+    assert counts, nts
+    cases = [(nb, bt, False, None) for nb in counts for bt in (1, 2, 4)]
+    cases += [(counts[-1], 0, True, None), (0, 0, True, None), (counts[0], 0, False, 1), (1, 0, True, counts[-1])]
+    for nb, bt, graphs, switch in cases:
+        label = f"blocks {nb} batch {bt} graphs {graphs} switch {switch}"
+        print(" ", label, flush=True)
+        e = run(SCHED_FUSED, nb, bt, graphs, switch)
+        assert e.schedule(0) == SCHED_FUSED
+        same(ref, e, label)
+        e.close()
+    ref.close()
+
+
+@pytest.mark.parametrize("prec", [CLV_FP32, CLV_FP64])
 def test_observables_identical_across_schedules_and_graphs(prec):
     """Observables (n_c, energy, count, F_a(x)) and fields are the same bit for
     bit for two-pass eager steps, fused eager steps and fused graph-replayed
