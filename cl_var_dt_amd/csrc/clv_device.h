@@ -102,9 +102,11 @@ __device__ __forceinline__ void normals4(double out[4], uint64_t seed, uint32_t 
   const double u2 = (double(r.z) + 0.5) * s, u3 = (double(r.w) + 0.5) * s;
   const double r0 = sd * sqrt(-2.0 * log(u0));
   const double r1 = sd * sqrt(-2.0 * log(u2));
+  // sin / cos of 2 pi u as sinpi / cospi of 2u (exact doubling, no 2 pi
+  // argument reduction): the same values to within the rounding of 2 pi u
   double s0, c0, s1, c1;
-  sincos(6.283185307179586 * u1, &s0, &c0);
-  sincos(6.283185307179586 * u3, &s1, &c1);
+  sincospi(2.0 * u1, &s0, &c0);
+  sincospi(2.0 * u3, &s1, &c1);
   out[0] = r0 * c0;
   out[1] = r0 * s0;
   out[2] = r1 * c1;
