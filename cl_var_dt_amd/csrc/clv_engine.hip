@@ -718,12 +718,16 @@ __global__ __launch_bounds__(256) void k_obs_finish(BucketArgs<T> A) {
   if (!evaluates(A, P, st)) return;
   const int64_t vs = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (vs < A.Vs) {
-    for (int a = 0; a < 3; ++a) {
-      double s = 0.0;
-      for (int c = 0; c < A.nchunks; ++c)
-        s += A.obs_stage[((int64_t(chain) * A.nchunks + c) * A.Vs + vs) * 3 + a];
-      A.obs_F[(int64_t(chain) * A.Vs + vs) * 3 + a] += s;
+    // the site's three values of each chunk together, chunks in order
+    double s[3] = {0.0, 0.0, 0.0};
+    for (int c = 0; c < A.nchunks; ++c) {
+      const double *o = A.obs_stage + ((int64_t(chain) * A.nchunks + c) * A.Vs + vs) * 3;
+#pragma unroll
+      for (int a = 0; a < 3; ++a) s[a] += o[a];
     }
+    double *F = A.obs_F + (int64_t(chain) * A.Vs + vs) * 3;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) F[a] += s[a];
   }
   if (blockIdx.x == 0 && threadIdx.x < 64) {
     const int lane = threadIdx.x;
