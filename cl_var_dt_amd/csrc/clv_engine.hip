@@ -2313,7 +2313,7 @@ int clv_set_fused_blocks(clv_ctx *ctx, int32_t blocks) {
   if (!ctx) return CLV_ERR_ARG;
   if (blocks < 0) return fail(ctx, CLV_ERR_ARG, "clv_set_fused_blocks: blocks < 0");
   for (const Bucket &b : ctx->buckets)
-    if (blocks > 0 && (b.Nt % blocks != 0 || b.Nt / blocks < 2))
+    if (blocks > 1 && (b.Nt % blocks != 0 || b.Nt / blocks < 2))
       return fail(ctx, CLV_ERR_ARG, "clv_set_fused_blocks: " + std::to_string(blocks) +
                                         " blocks do not divide Nt = " + std::to_string(b.Nt) +
                                         " into blocks of at least 2 slices");

@@ -213,8 +213,9 @@ int clv_set_fused_options(clv_ctx *ctx, int32_t line_sets, int32_t batch);
  * blocks, one workgroup per (row tile, chain, block), so few chains still
  * give enough workgroups; the first slice of each block gets its next-step
  * |D_0| partial from a short drift-statistics pass after the march.
- * blocks must divide Nt into blocks of at least 2 slices; 0 = the untimed
- * choice (the fewest blocks of at least 8 slices that give 512 workgroups).
+ * blocks > 1 must divide Nt into blocks of at least 2 slices; 1 = one block
+ * (the whole tau range); 0 = the untimed choice (the fewest blocks of at
+ * least 8 slices that give 512 workgroups).
  * Results do not depend on it; clv_autotune times it. */
 int clv_set_fused_blocks(clv_ctx *ctx, int32_t blocks);
 /* clv_propagate replays captured HIP graphs of its steps (on by default): a
