@@ -10,6 +10,7 @@ the fused march over long tau ranges are compared with the oracle directly:
   workload                             tiles x Nt per chain   steps   bar
   128 chains of 128x128x1x32, FP32     64 x 32                50      1e-4 rms-relative per site
   one 128x128x1x32 chain, FP64         64 x 32                3       1e-11 per site, both schedules
+  16 chains of 64x64x1x64, FP64        16 x 64, 2 tau blocks  50      1e-10 per site, fused in tau blocks
   2-D 64x64x1x64, FP64 (device dt)     16 x 64                1000    1e-10 per site (the north-star gate)
   uneven 3-D batch, 24^3 / 32^3, FP32  144 / 128 x Nt          20      1e-4 rms-relative per site
   256x256x1x128, FP32 (context+server) 256 x 128              10      1e-4 rms-relative; observables 1e-4
@@ -114,6 +115,42 @@ def test_128x128x32_fp64_steps_match_oracle(oracle_lib, sched):
     assert sg.dt == pytest.approx(so.dt, rel=1e-12)
     assert sg.max_drift == pytest.approx(so.max_drift, rel=1e-12)
     assert sg.mean_drift == pytest.approx(so.mean_drift, rel=1e-12)
+    e.close()
+
+
+def test_fp64_batch_64x64x64_tau_blocks_match_oracle(oracle_lib):
+    """The FP64 batched workload (16 chains of 64x64x1x64, the bench's
+    fp64_batched leg) on its untimed production plan -- the fused march in two
+    tau blocks, the blocks' first slices completed by the strided
+    drift-statistics pass, graph-replayed steps with observables every 10 --
+    against the oracle for chains 0 and 15 after 50 steps: 1e-10 per site,
+    dt and the drift statistics to 1e-12, observables to 1e-10."""
+    print("case: 16 x 64x64x1x64 fp64 tau blocks vs oracle", flush=True)
+    from cl_var_dt_amd._abi import SCHED_FUSED
+    shape = (64, 64, 1, 64)
+    nch = 16
+    ps = [_params(300 + i, 64, mu=0.5 + i / (nch - 1)) for i in range(nch)]
+    e = _engine([shape] * nch, CLV_FP64)
+    for i in range(nch):
+        e.set_params(i, ps[i])
+        e.init_fields(i, 1.0, 0.1)
+    e.set_schedule(0)
+    assert e.schedule(0) == SCHED_FUSED  # 16 tiles x 16 chains x 2 blocks = 512 workgroups
+    e.propagate(50, 10)
+    for i in (0, nch - 1):
+        (c, d), so, oo = _oracle(oracle_lib, shape, ps[i], 50, 10)
+        a, b = e.get_fields(i)
+        err = max(np.max(abs(a - c)), np.max(abs(b - d)))
+        sg = e.stats(i)
+        o, _ = e.observables(i)
+        print(f"chain {i}: max per-site |delta| = {err:.3e}", flush=True)
+        assert err <= 1e-10
+        assert sg.dt == pytest.approx(so.dt, rel=1e-12)
+        assert sg.max_drift == pytest.approx(so.max_drift, rel=1e-12)
+        assert sg.mean_drift == pytest.approx(so.mean_drift, rel=1e-12)
+        assert o.count == oo.count
+        assert o.n_tot == pytest.approx(oo.n_tot, rel=1e-10)
+        assert o.energy == pytest.approx(oo.energy, rel=1e-10)
     e.close()
 
 
