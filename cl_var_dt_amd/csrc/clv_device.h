@@ -7,6 +7,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "clv_math64.h"
+
 namespace clv {
 
 // ---------------------------------------------------------------- complex --
@@ -91,7 +93,8 @@ __device__ __forceinline__ uint4 philox10(uint4 c, uint32_t k0, uint32_t k1) {
 // Two complex N(0, sd^2) draws for (seed, stream, step, site, pair):
 // counter (site, pair | stream << 8, step_lo, step_hi), uniforms
 // (x + 1/2) 2^-32, Box-Muller (1958) on (u0,u1) and (u2,u3), the standard
-// deviation sd folded into the radius.
+// deviation sd folded into the radius.  The logarithm and the sine/cosine
+// are the uniform-range routines of clv_math64.h (within 2 ulp).
 __device__ __forceinline__ void normals4(double out[4], uint64_t seed, uint32_t stream,
                                          uint64_t step, uint32_t site, uint32_t pair,
                                          double sd = 1.0) {
@@ -100,13 +103,11 @@ __device__ __forceinline__ void normals4(double out[4], uint64_t seed, uint32_t 
   const double s = 2.3283064365386963e-10;  // 2^-32
   const double u0 = (double(r.x) + 0.5) * s, u1 = (double(r.y) + 0.5) * s;
   const double u2 = (double(r.z) + 0.5) * s, u3 = (double(r.w) + 0.5) * s;
-  const double r0 = sd * sqrt(-2.0 * log(u0));
-  const double r1 = sd * sqrt(-2.0 * log(u2));
-  // sin / cos of 2 pi u as sinpi / cospi of 2u (exact doubling, no 2 pi
-  // argument reduction): the same values to within the rounding of 2 pi u
+  const double r0 = sd * sqrt(-2.0 * log_unit(u0));
+  const double r1 = sd * sqrt(-2.0 * log_unit(u2));
   double s0, c0, s1, c1;
-  sincospi(2.0 * u1, &s0, &c0);
-  sincospi(2.0 * u3, &s1, &c1);
+  sincos_2pi(u1, &s0, &c0);
+  sincos_2pi(u3, &s1, &c1);
   out[0] = r0 * c0;
   out[1] = r0 * s0;
   out[2] = r1 * c1;
