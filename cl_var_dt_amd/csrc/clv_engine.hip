@@ -1247,7 +1247,13 @@ void plan_bucket(Bucket &b, int precision) {
   b.S = b.Vs * sh.nt;
   b.esz = (precision == CLV_FP32) ? 8 : 16;
   b.n1 = default_n1(b.X);
-  b.rows = pick_rows(b.X, b.R, std::max(kTileSites, b.X));
+  // FP64 rows of up to 64 sites in tiles of 128 sites (two waves): at two
+  // waves/SIMD the smaller workgroups wait less at their barriers.  Measured
+  // at 64x64x1x64: fused step 3% and two-pass step 5-10% faster than in tiles
+  // of 256; at 32x32x1x32 1-4%; longer FP64 rows keep 256 (one 128-site row
+  // per workgroup made the fused step at 128x128x1x32 16% slower).
+  const int tile_sites = (precision == CLV_FP64 && b.X <= 64) ? kTileSites / 2 : kTileSites;
+  b.rows = pick_rows(b.X, b.R, std::max(tile_sites, b.X));
   // the update kernel stages 8 lines per row: keep its LDS within 64 KB (the
   // row tile follows the default split only, so no split changes the tiling)
   while (b.rows > 1 && update_lds_bytes(b, b.rows, b.n1) > 65536) {
@@ -2159,7 +2165,7 @@ int restore_snapshot(clv_ctx *ctx, Bucket &b, Snapshot &sn, bool release) {
 // 64 KB of LDS at the bucket's row tile, and for each the reduction tiles that
 // divide Nt and fit LDS, and the fused line sets and K_0 batches (of at most
 // Nt) that fit 64 KB, each with one tau block and, where the chains give
-// fewer than kFusedMinWorkgroups workgroups, with the untimed block count.
+// fewer than kFusedMinWaves waves, with the untimed block count.
 std::vector<TuneChoice> tune_choices(const Bucket &b) {
   std::vector<TuneChoice> out;
   for (int n1 : split_options(b.X)) {

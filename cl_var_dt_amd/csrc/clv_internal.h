@@ -73,17 +73,19 @@ inline bool uses_fused(const Bucket &b) { return b.schedule == 2 && b.fused_lds 
 // the two-pass time across 2-D and 3-D shapes and Nt = 16..128, at 320-384 it
 // is 1.05-1.33.  FP64 (two waves/SIMD, no spills): 0.80-0.81 of the two-pass
 // step at 512-1024 workgroups (32 and 64 chains of 64x64x1x64, 16 chains of
-// 128x128x1x32), equal at 256 (16 chains of 64x64x1x64).  Both schedules give
-// identical results, so the choice may change at any step.
-constexpr int kFusedMinWorkgroups = 512;
+// 128x128x1x32), equal at 256 (16 chains of 64x64x1x64).  Counted in waves
+// (those were 4-wave workgroups), so two-wave workgroups need twice as many.
+// Both schedules give identical results, so the choice may change at any step.
+constexpr int64_t kFusedMinWaves = 2048;
 // Tau blocks of the fused march for nactive chains: the fewest (a power of two
 // dividing Nt, blocks of at least kFusedMinBlock slices) that reach
-// kFusedMinWorkgroups workgroups, 0 when none does.
+// kFusedMinWaves waves, 0 when none does.
 constexpr int kFusedMinBlock = 8;
 inline int fused_blocks_for(const Bucket &b, int nactive) {
+  const int64_t waves = int64_t(b.ntiles) * nactive * ((b.nthr + 63) / 64);
   for (int nb = 1; nb <= b.Nt / kFusedMinBlock || nb == 1; nb *= 2) {
     if (b.Nt % nb) break;
-    if (int64_t(b.ntiles) * nactive * nb >= kFusedMinWorkgroups) return nb;
+    if (waves * nb >= kFusedMinWaves) return nb;
   }
   return 0;
 }
