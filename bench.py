@@ -71,7 +71,7 @@ def measured_valu(kernel: str, x: int, lattice: str, chains: int):
     return None, None
 
 
-def run_batched(args, log, fp64=False, fp64_chains=16):
+def run_batched(args, log, fp64=False, fp64_chains=16, steps=None, warmup=None):
     import torch
     from cl_var_dt_amd import CLV_FP32, CLV_FP64, Shape
     from cl_var_dt_amd.engine import Engine
@@ -100,12 +100,14 @@ def run_batched(args, log, fp64=False, fp64_chains=16):
         tuning = lines[0] if lines else None
     log(f"batched: schedule {eng.schedule(0)}, {nch} chains of {X}x{Y}x1x{Nt} {'FP64' if fp64 else 'FP32'}, "
         f"device bytes {eng.device_bytes()/1e9:.2f} GB")
-    eng.propagate(args.warmup, 10)
+    steps = args.steps if steps is None else steps
+    eng.propagate(args.warmup if warmup is None else warmup, 10)
+    eng.prepare_graph(10)  # the graph capture (one-time set-up) stays out of the timed region
     eng.synchronize()
     torch.cuda.synchronize()
     # timed region: the production path (graph-replayed steps, no kernel events)
     t0 = time.perf_counter()
-    eng.propagate(args.steps, 10)
+    eng.propagate(steps, 10)
     eng.synchronize()
     torch.cuda.synchronize()
     t1 = time.perf_counter()
@@ -124,7 +126,7 @@ def run_batched(args, log, fp64=False, fp64_chains=16):
     bad = sum(eng.stats(i).nonfinite for i in range(nch))
     sched = eng.schedule(0)
     eng.close()
-    return dict(sites=sites, wall=wall, su=sites * args.steps / wall, kt=kt, dt0=st.dt, nonfinite=bad,
+    return dict(sites=sites, wall=wall, su=sites * steps / wall, steps=steps, kt=kt, dt0=st.dt, nonfinite=bad,
                 schedule=sched, tuning=tuning, prof_su=sites * args.prof_steps / prof_wall)
 
 
@@ -137,7 +139,7 @@ def run_plain(args, log):
     eng = Engine([shape], precision=CLV_FP64)
     eng.set_params(0, _params(7, nt=64))
     eng.init_fields(0, 1.0, 0.1)
-    eng.propagate_plain(0, 10, 10)
+    eng.propagate_plain(0, args.plain_warmup, 10)
     eng.synchronize()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -175,6 +177,7 @@ def run_uneven(args, log):
         eng.init_fields(i, 1.0, 0.1)
     eng.autotune(args.tune_cache)
     eng.propagate(args.warmup, 10)
+    eng.prepare_graph(10)  # the graph capture (one-time set-up) stays out of the timed region
     eng.synchronize()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -339,6 +342,8 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=30)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--plain-steps", type=int, default=300)
+    ap.add_argument("--plain-warmup", type=int, default=200,
+                    help="untimed steps of the plain FP64 leg before its timed window")
     ap.add_argument("--chains", type=int, default=128)
     ap.add_argument("--lx", type=int, default=128)
     ap.add_argument("--nt", type=int, default=32)
@@ -347,6 +352,10 @@ def main() -> int:
                     help="FP64 batched leg: comma-separated chain counts (0: skip the leg)")
     ap.add_argument("--fp64-lx", type=int, default=64)
     ap.add_argument("--fp64-nt", type=int, default=64)
+    ap.add_argument("--fp64-warmup", type=int, default=400,
+                    help="FP64 batched legs: at least this many untimed steps before the timed window")
+    ap.add_argument("--fp64-steps", type=int, default=300,
+                    help="FP64 batched legs: at least this many timed steps")
     ap.add_argument("--precision", choices=["fp32", "fp64"], default="fp32",
                     help="--mode batched: precision of the batched leg (fp64 uses --fp64-*)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -406,15 +415,21 @@ def main() -> int:
         print(json.dumps({"batched_su_per_s": b["su"], "kernel_ms": b["kt"]}), flush=True)
         return 0
     # the FP64 batched path: chains of the plain path's lattice advanced together
-    # (autotuned schedule), against 256 algorithmic bytes per site-update
+    # (autotuned schedule), against 256 algorithmic bytes per site-update.  Its
+    # steps are short (about 0.3 ms at 16 chains) and the GPU's clocks take
+    # about 100 ms of steady load after the autotuner to settle (the first 30
+    # steps run about 12% slower than the 300th), so it warms up and times
+    # windows of at least --fp64-warmup and --fp64-steps steps.
     b64 = []
     for nc in fp64_counts:
-        r = run_batched(args, log, fp64=True, fp64_chains=nc)
-        step_s = r["wall"] / args.steps
+        r = run_batched(args, log, fp64=True, fp64_chains=nc, steps=max(args.steps, args.fp64_steps),
+                        warmup=max(args.warmup, args.fp64_warmup))
+        step_s = r["wall"] / r["steps"]
         t = r["tuning"] or {}
         b64.append({"value": round(r["su"], 1), "unit": "site-updates/s", "chains": nc,
                     "lattice": f"{args.fp64_lx}x{args.fp64_lx}x1x{args.fp64_nt}", "schedule": r["schedule"],
                     "tau_blocks": t.get("fblocks"), "ms_per_step": round(1e3 * step_s, 4),
+                    "steps": r["steps"], "warmup": max(args.warmup, args.fp64_warmup),
                     "hbm_frac": round(256 * r["sites"] / step_s / 1e9 / HBM_PEAK_GBS, 4),
                     "workload": "FP64 fields and reductions, per-chain adaptive dt, observables every 10 steps, "
                                 "autotuned, graph-replayed steps; hbm_frac = 256 algorithmic bytes per "
@@ -476,6 +491,7 @@ def main() -> int:
         },
         "fp64_plain": {"value": round(p["su"], 1), "unit": "site-updates/s",
                        "ms_per_step": round(p["ms_per_step"], 4),
+                       "steps": args.plain_steps, "warmup": args.plain_warmup,
                        "workload": "one 2-D 64x64 x Nt=64 chain, FP64, host-driven dt per step"},
         "batched_over_plain": round(b["su"] / p["su"], 2),
         "fp64_batched": b64 or None,

@@ -162,6 +162,7 @@ def _declare(lib):
         "clv_set_fused_options": (C.c_int, [vp, i32, i32]),
         "clv_set_fused_blocks": (C.c_int, [vp, i32]),
         "clv_set_graphs": (C.c_int, [vp, i32]),
+        "clv_prepare_graph": (C.c_int, [vp, i32]),
         "clv_set_profiling": (C.c_int, [vp, i32]),
         "clv_kernel_time": (C.c_int, [vp, i32, P(dbl), P(i64)]),
         "clv_step_noalloc": (C.c_int, [P(Shape), i32, P(Params), vp, vp, vp, vp, P(StepStats), i32, vp]),
@@ -193,7 +194,7 @@ def exported_symbols() -> list[str]:
         "clv_get_step_stats", "clv_set_step_stats", "clv_synchronize", "clv_evaluate", "clv_read_observables",
         "clv_reset_observables", "clv_autotune", "clv_set_schedule", "clv_get_schedule",
         "clv_set_row_split", "clv_get_row_split",
-        "clv_set_fused_options", "clv_set_fused_blocks", "clv_set_profiling", "clv_kernel_time", "clv_set_graphs", "clv_step_noalloc",
+        "clv_set_fused_options", "clv_set_fused_blocks", "clv_set_profiling", "clv_kernel_time", "clv_set_graphs", "clv_prepare_graph", "clv_step_noalloc",
         "clv_server_create", "clv_server_destroy", "clv_server_submit", "clv_server_poll",
         "clv_server_wait", "clv_server_release", "clv_server_cancel", "clv_server_info_get", "clv_server_clock", "clv_server_error",
     ]

@@ -1878,6 +1878,15 @@ int clv_propagate(clv_ctx *ctx, int64_t nsteps, int32_t eval_every) {
   return CLV_OK;
 }
 
+int clv_prepare_graph(clv_ctx *ctx, int32_t eval_every) {
+  if (!ctx) return CLV_ERR_ARG;
+  if (!ctx->use_graphs || ctx->profiling) return CLV_OK;
+  (void)hipSetDevice(ctx->device);
+  hipGraphExec_t exec = nullptr;
+  int G = 2;
+  return graph_for(ctx, eval_every, &exec, &G);
+}
+
 int clv_drift_stats(clv_ctx *ctx, int32_t chain, double *max_drift, double *mean_drift) {
   int rc = check_chain(ctx, chain);
   if (rc) return rc;

@@ -160,6 +160,11 @@ class Engine:
     def set_graphs(self, on: bool) -> None:
         self._chk(lib().clv_set_graphs(self._h, 1 if on else 0))
 
+    def prepare_graph(self, eval_every: int = 0) -> None:
+        """Capture the step graph the next propagate(n, eval_every) replays,
+        without running it (keeps the one-time capture out of a timed region)."""
+        self._chk(lib().clv_prepare_graph(self._h, eval_every))
+
     # ---- kernel timing (HIP events on the context's stream) ----
     def set_profiling(self, on: bool) -> None:
         self._chk(lib().clv_set_profiling(self._h, 1 if on else 0))

@@ -215,7 +215,7 @@ int clv_set_fused_options(clv_ctx *ctx, int32_t line_sets, int32_t batch);
  * |D_0| partial from a short drift-statistics pass after the march.
  * blocks > 1 must divide Nt into blocks of at least 2 slices; 1 = one block
  * (the whole tau range); 0 = the untimed choice (the fewest blocks of at
- * least 8 slices that give 512 workgroups).
+ * least 8 slices that give 2048 waves, e.g. 512 four-wave workgroups).
  * Results do not depend on it; clv_autotune times it. */
 int clv_set_fused_blocks(clv_ctx *ctx, int32_t blocks);
 /* clv_propagate replays captured HIP graphs of its steps (on by default): a
@@ -224,6 +224,13 @@ int clv_set_fused_blocks(clv_ctx *ctx, int32_t blocks);
  * with the observable kernels gated per chain; off runs every step eagerly.
  * Results are identical. */
 int clv_set_graphs(clv_ctx *ctx, int32_t on);
+/* Capture and instantiate the step graph that the next clv_propagate with
+ * this eval_every would replay (for the buckets' current schedules, state
+ * slots and cadence phases), without launching it (a project extension with no
+ * counterpart in the reference: for callers that keep the one-time capture out
+ * of a timed or latency-critical region).  No-op with graphs off or kernel
+ * timing on; a later change of schedule or blocks drops it. */
+int clv_prepare_graph(clv_ctx *ctx, int32_t eval_every);
 
 /* ---- kernel timing -----------------------------------------------------
  * With profiling on, every kernel the context launches is bracketed by HIP

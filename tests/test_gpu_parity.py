@@ -633,6 +633,47 @@ def test_graphs_with_chains_at_different_steps(sched):
         e.close()
 
 
+@pytest.mark.parametrize("sched", [1, 2])
+def test_prepared_graph_replays_like_eager_steps(sched):
+    """clv_prepare_graph captures, without running it, the graph that the next
+    propagate with that cadence replays (here at cadence phase 3, after 3 eager
+    warm-up steps); the replay then gives eager steps' fields, stats and
+    observables bit for bit (evaluations at steps 0, 10 and 20), and preparing
+    twice is harmless."""
+    print("case: prepared graph", sched, flush=True)
+    shapes = [(32, 4, 1, 8)] * 3
+    engines = []
+    for graphs in (True, False):
+        e = _engine(shapes, CLV_FP32)
+        for i in range(3):
+            e.set_params(i, _params(seed=90 + i, mu=0.9))
+            e.init_fields(i, 1.0, 0.1)
+        e.set_schedule(sched)
+        e.set_graphs(graphs)
+        engines.append(e)
+    g, eager = engines
+    for e in engines:
+        e.propagate(3, 10)
+    before = [g.stats(i).steps for i in range(3)]
+    g.prepare_graph(10)
+    g.prepare_graph(10)
+    assert [g.stats(i).steps for i in range(3)] == before == [3, 3, 3]
+    g.propagate(20, 10)
+    eager.propagate(20, 10)
+    for i in range(3):
+        a, b = g.get_fields(i)
+        c, d = eager.get_fields(i)
+        assert np.array_equal(a, c) and np.array_equal(b, d), i
+        sg, se = g.stats(i), eager.stats(i)
+        assert (sg.steps, sg.dt, sg.max_drift, sg.mean_drift, sg.t_langevin) == \
+               (se.steps, se.dt, se.max_drift, se.mean_drift, se.t_langevin), i
+        oa, _ = g.observables(i)
+        ob, _ = eager.observables(i)
+        assert oa.count == ob.count == 3 and oa.n_tot == ob.n_tot and oa.energy == ob.energy, i
+    for e in engines:
+        e.close()
+
+
 def test_autotune_keeps_state_and_caches(tmp_path):
     """The autotuner times the row splits, schedules, reduction tiles and fused
     LDS options on a snapshot of the live state and restores it; the choice is
