@@ -208,6 +208,16 @@ def lib():
             raise ImportError(
                 f"{LIB_PATH} is missing: build it with `python __graft_entry__.py` "
                 "(hipcc --offload-arch=gfx950); there is no CPU fallback")
+        # PyTorch's ROCm wheels bring their own HIP and HSA runtimes.  A process
+        # may hold only one working HSA runtime: when torch is loaded first,
+        # libclv's HIP runtime binds to torch's already-loaded HSA runtime (the
+        # same soname), and both see the GPU; when libclv is loaded first, the
+        # two HSA runtimes differ and whichever initialises second finds no
+        # device.  So torch, where installed, is loaded before libclv.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         _lib = _declare(C.CDLL(LIB_PATH))
     return _lib
 

@@ -1,9 +1,11 @@
-"""CPU-side checks of the C ABI: libclv.so loads, exports every entry point
-that include/clv.h declares, and its host-only functions behave (no GPU call)."""
+"""Checks of the C ABI: libclv.so loads, exports every entry point that
+include/clv.h declares, and its host-only functions behave (no GPU call); on
+a GPU, torch and libclv both see the device whichever is used first."""
 import ctypes as C
 import os
 import re
 import subprocess
+import sys
 
 import pytest
 
@@ -98,3 +100,41 @@ def test_create_rejects_bad_shapes(libclv):
     shp = (_abi.Shape * 1)(_abi.Shape(8, 1, 1, 4))
     assert libclv.clv_create(shp, 1, 7, 0, C.byref(h)) == _abi.CLV_ERR_ARG
     assert b"precision" in libclv.clv_last_error(None)
+
+
+_ORDER = r'''
+import ctypes as C, sys
+sys.path.insert(0, {root!r})
+from cl_var_dt_amd import _abi
+def create():
+    shp = (_abi.Shape * 1)(_abi.Shape(8, 1, 1, 4)); h = C.c_void_p()
+    st = _abi.lib().clv_create(shp, 1, _abi.CLV_FP64, 0, C.byref(h))
+    if st == 0:
+        _abi.lib().clv_destroy(h)
+    return st
+if {case!r} == "lib-first":
+    _abi.lib()
+    import torch
+    ok = torch.cuda.is_available() and float(torch.ones(4, device="cuda").sum()) == 4.0
+    print(int(ok), create())
+else:
+    st = create()
+    import torch
+    ok = torch.cuda.is_available() and float(torch.ones(4, device="cuda").sum()) == 4.0
+    print(int(ok), st)
+'''
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", ["lib-first", "create-first"])
+def test_torch_and_libclv_share_the_gpu_in_any_order(case):
+    """A Python process that uses both torch (ROCm wheel, its own HIP/HSA
+    runtimes) and libclv sees the GPU from both, whichever it touches first:
+    loading libclv first used to leave the second runtime without a device."""
+    print("case: runtime order", case, flush=True)
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    out = subprocess.run([sys.executable, "-c", _ORDER.format(root=root, case=case)],
+                         capture_output=True, text=True, timeout=240)
+    assert out.returncode == 0, out.stderr[-2000:]
+    torch_ok, st = out.stdout.split()[-2:]
+    assert torch_ok == "1" and st == "0", out.stdout
