@@ -10,8 +10,8 @@ the fused march over long tau ranges are compared with the oracle directly:
   workload                             tiles x Nt per chain   steps   bar
   128 chains of 128x128x1x32, FP32     64 x 32                50      1e-4 rms-relative per site
   one 128x128x1x32 chain, FP64         64 x 32                3       1e-11 per site, both schedules
-  16 chains of 64x64x1x64, FP64        16 x 64, 2 tau blocks  50      1e-10 per site, fused in tau blocks
-  2-D 64x64x1x64, FP64 (device dt)     16 x 64                1000    1e-10 per site (the north-star gate)
+  16 chains of 64x64x1x64, FP64        32 x 64, 2 tau blocks  50      1e-10 per site, fused in tau blocks
+  2-D 64x64x1x64, FP64 (device dt)     32 x 64                1000    1e-10 per site (the 1000-step trajectory gate)
   uneven 3-D batch, 24^3 / 32^3, FP32  144 / 128 x Nt          20      1e-4 rms-relative per site
   256x256x1x128, FP32 (context+server) 256 x 128              10      1e-4 rms-relative; observables 1e-4
 
@@ -135,7 +135,7 @@ def test_fp64_batch_64x64x64_tau_blocks_match_oracle(oracle_lib):
         e.set_params(i, ps[i])
         e.init_fields(i, 1.0, 0.1)
     e.set_schedule(0)
-    assert e.schedule(0) == SCHED_FUSED  # 16 tiles x 16 chains x 2 blocks = 512 workgroups
+    assert e.schedule(0) == SCHED_FUSED  # 32 two-wave tiles x 16 chains x 2 blocks = 2048 waves
     e.propagate(50, 10)
     for i in (0, nch - 1):
         (c, d), so, oo = _oracle(oracle_lib, shape, ps[i], 50, 10)

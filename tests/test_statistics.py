@@ -12,7 +12,7 @@
    against FP64 engine chains, on the same Philox streams (chain i has seed i
    in both).  Two checks per observable, each printed:
    - statistical: the two means agree within two standard errors of the
-     independent-chain spread (the north-star criterion);
+     independent-chain spread (the acceptance criterion for FP32 observables);
    - paired: the per-chain differences d_i = fp32_i - fp64_i have a mean
      within two standard errors of the d_i themselves plus the FP32 rounding
      allowance 1e-5 (|mean| + chain spread) -- a bias ten times the observed
