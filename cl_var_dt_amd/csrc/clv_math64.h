@@ -11,11 +11,12 @@
 //              log u = e ln2 + log m with ln2 split into two doubles;
 //   sincos_2pi: 2 pi v = (pi/2) j + pi r with j = rint(4v) and |r| <= 1/4
 //              (exact in binary), sin(pi r) and cos(pi r) by their Taylor
-//              series to r^17 and r^16 (the first omitted terms are below
-//              1e-18), then the quarter-turn rotation by j.
-// Every multiply-add is an explicit fma and nothing else is contracted, so
-// the host and the device compute the same bits; the CPU tests check the
-// error against long-double references on both paths' code.
+//              series to r^17 and r^16 (the first omitted terms are at
+//              most 2e-18), then the quarter-turn rotation by j.
+// Every multiply-add is an explicit fma and nothing else is contracted, and
+// the divide, frexp, rint and fma are IEEE-exact on both sides, so a host
+// build performs the device's operation sequence: tests/test_math64.py checks
+// that build against long-double references.
 #pragma once
 
 #include <math.h>
